@@ -1,0 +1,137 @@
+// Gather pass: per drawable handle resolution and indirect-command /
+// DrawablePointers fill.
+//
+// Semantics follow src/CadR/shaders/processDrawables.comp:92-113 (main) and
+// :77-89 (lookupHandle); the launch replaces the 32768 x Y split dispatch of
+// src/CadR/Renderer.cpp:684-692 with one 1-D grid of wave64-friendly blocks
+// (the split only works around Vulkan's per-dimension workgroup-count limit,
+// the drawable -> index mapping is the same flat index).
+//
+// gfx950 shape: one lane per drawable, 256 lanes per block (4 waves), the
+// handle-table depth a template parameter so each variant is a straight-line
+// chain of dependent 8-byte loads.  The 48-byte record is read as three
+// 16-byte loads, the 16-byte command and 32-byte pointer records are written
+// with 16-byte stores; consecutive lanes touch consecutive records, so every
+// wave reads and writes whole contiguous spans.
+#include "internal.h"
+
+namespace cadr {
+
+struct alignas(16) DrawableRecord {
+	uint64_t vertexDataHandle;
+	uint64_t indexDataHandle;
+	uint64_t matrixListHandle;
+	uint64_t drawableDataHandle;
+	uint64_t primitiveSetHandle;
+	uint32_t primitiveSetOffset;
+	uint32_t padding;
+};
+static_assert(sizeof(DrawableRecord) == 48, "DrawableGpuData is 48 bytes (Drawable.cpp:13)");
+
+// Handle -> device address through 1, 2 or 3 tables of 2048 uint64 entries
+// (processDrawables.comp:77-89: level 1 table[h], level 2 table[h>>11][h&0x7ff],
+// level 3 table[h>>22][(h>>11)&0x7ff][h&0x7ff], indices truncated to 32 bits
+// as uint(handle) does in GLSL).  An index that falls outside a table, or a
+// table pointer that is 0, resolves to 0 instead of reading unowned memory.
+template <int LEVEL>
+__device__ __forceinline__ uint64_t lookupHandle(const uint64_t* __restrict__ root, uint64_t handle)
+{
+	const uint32_t h = uint32_t(handle);
+	if constexpr (LEVEL == 1) {
+		if (h >= CADR_HANDLES_PER_TABLE) return 0;
+		return root[h];
+	} else if constexpr (LEVEL == 2) {
+		const uint32_t i1 = uint32_t(handle >> CADR_HANDLE_LEVEL_SHIFT);
+		if (i1 >= CADR_HANDLES_PER_TABLE) return 0;
+		const uint64_t* t2 = reinterpret_cast<const uint64_t*>(root[i1]);
+		if (!t2) return 0;
+		return t2[h & CADR_HANDLE_LEVEL_MASK];
+	} else {
+		const uint32_t i1 = uint32_t(handle >> (2 * CADR_HANDLE_LEVEL_SHIFT));
+		if (i1 >= CADR_HANDLES_PER_TABLE) return 0;
+		const uint64_t* t2 = reinterpret_cast<const uint64_t*>(root[i1]);
+		if (!t2) return 0;
+		const uint64_t* t3 = reinterpret_cast<const uint64_t*>(
+		    t2[uint32_t(handle >> CADR_HANDLE_LEVEL_SHIFT) & CADR_HANDLE_LEVEL_MASK]);
+		if (!t3) return 0;
+		return t3[h & CADR_HANDLE_LEVEL_MASK];
+	}
+}
+
+template <int LEVEL>
+__global__ __launch_bounds__(256) void processDrawablesKernel(const uint64_t* __restrict__ handleTable,
+                                                              const uint4* __restrict__ drawableList,
+                                                              uint4* __restrict__ indirectData,
+                                                              uint4* __restrict__ drawablePointers,
+                                                              uint64_t numDrawables)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+	if (i >= numDrawables) return;
+
+	// DrawableGpuData as three 16-byte words
+	const uint4 w0 = drawableList[3 * i + 0];  // vertexDataHandle, indexDataHandle
+	const uint4 w1 = drawableList[3 * i + 1];  // matrixListHandle, drawableDataHandle
+	const uint4 w2 = drawableList[3 * i + 2];  // primitiveSetHandle, primitiveSetOffset, padding
+	const uint64_t vertexH = uint64_t(w0.x) | (uint64_t(w0.y) << 32);
+	const uint64_t indexH = uint64_t(w0.z) | (uint64_t(w0.w) << 32);
+	const uint64_t matrixListH = uint64_t(w1.x) | (uint64_t(w1.y) << 32);
+	const uint64_t drawableDataH = uint64_t(w1.z) | (uint64_t(w1.w) << 32);
+	const uint64_t primitiveSetH = uint64_t(w2.x) | (uint64_t(w2.y) << 32);
+	const uint32_t primitiveSetOffset = w2.z;
+
+	// PrimitiveSet {count, first} and MatrixList header
+	const uint64_t psBase = lookupHandle<LEVEL>(handleTable, primitiveSetH);
+	const uint64_t ml = lookupHandle<LEVEL>(handleTable, matrixListH);
+	uint32_t psCount = 0, psFirst = 0, numMatrices = 0;
+	if (psBase) {
+		const uint32_t* ps = reinterpret_cast<const uint32_t*>(psBase + primitiveSetOffset);
+		psCount = ps[0];
+		psFirst = ps[1];
+	}
+	if (ml) numMatrices = *reinterpret_cast<const uint32_t*>(ml);
+
+	// VkDrawIndirectCommand {vertexCount, instanceCount, firstVertex, firstInstance}
+	indirectData[i] = make_uint4(psCount, numMatrices, psFirst, 0u);
+
+	// DrawablePointers {vertexDataPtr, indexDataPtr, matrixListPtr, drawableDataPtr}
+	const uint64_t vertexPtr = lookupHandle<LEVEL>(handleTable, vertexH);
+	const uint64_t indexPtr = lookupHandle<LEVEL>(handleTable, indexH);
+	const uint64_t drawableDataPtr = lookupHandle<LEVEL>(handleTable, drawableDataH);
+	drawablePointers[2 * i + 0] =
+	    make_uint4(uint32_t(vertexPtr), uint32_t(vertexPtr >> 32), uint32_t(indexPtr), uint32_t(indexPtr >> 32));
+	drawablePointers[2 * i + 1] = make_uint4(uint32_t(ml), uint32_t(ml >> 32), uint32_t(drawableDataPtr),
+	                                         uint32_t(drawableDataPtr >> 32));
+}
+
+}  // namespace cadr
+
+using namespace cadr;
+
+CADR_API int cadrProcessDrawables(const cadrProcessDrawablesPushConstants* pc, uint32_t handleLevel,
+                                  uint64_t numDrawables, cadrStream stream)
+{
+	CADR_REQUIRE(pc, "cadrProcessDrawables: pushConstants is NULL");
+	if (numDrawables == 0) return ok();  // Renderer.cpp:600-620 returns early
+	CADR_REQUIRE(numDrawables <= CADR_MAX_DRAWABLES,
+	             "cadrProcessDrawables: numDrawables exceeds the 1Gi limit (Renderer.cpp:687)");
+	CADR_REQUIRE(handleLevel >= 1 && handleLevel <= 3, "cadrProcessDrawables: handleLevel must be 1, 2 or 3");
+	CADR_REQUIRE(pc->handleTable && pc->drawableList && pc->indirectData && pc->drawablePointers,
+	             "cadrProcessDrawables: null device address in push constants");
+	CADR_REQUIRE((pc->handleTable % 8) == 0, "cadrProcessDrawables: handleTable must be 8-byte aligned");
+	CADR_REQUIRE((pc->drawableList % 16) == 0 && (pc->indirectData % 16) == 0 && (pc->drawablePointers % 16) == 0,
+	             "cadrProcessDrawables: drawableList, indirectData and drawablePointers must be 16-byte aligned");
+
+	const uint64_t blocks = (numDrawables + 255) / 256;
+	const auto* table = reinterpret_cast<const uint64_t*>(pc->handleTable);
+	const auto* list = reinterpret_cast<const uint4*>(pc->drawableList);
+	auto* indirect = reinterpret_cast<uint4*>(pc->indirectData);
+	auto* pointers = reinterpret_cast<uint4*>(pc->drawablePointers);
+	hipStream_t s = hipStream_t(stream);
+	switch (handleLevel) {
+	case 1: hipLaunchKernelGGL(processDrawablesKernel<1>, dim3(uint32_t(blocks)), dim3(256), 0, s, table, list, indirect, pointers, numDrawables); break;
+	case 2: hipLaunchKernelGGL(processDrawablesKernel<2>, dim3(uint32_t(blocks)), dim3(256), 0, s, table, list, indirect, pointers, numDrawables); break;
+	default: hipLaunchKernelGGL(processDrawablesKernel<3>, dim3(uint32_t(blocks)), dim3(256), 0, s, table, list, indirect, pointers, numDrawables); break;
+	}
+	CADR_CHECK_HIP(hipGetLastError(), "processDrawablesKernel launch");
+	return ok();
+}

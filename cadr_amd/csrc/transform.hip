@@ -1,0 +1,132 @@
+// Transform propagation by tree level and world bounding spheres.
+//
+// The reference propagates a node hierarchy on the CPU by recursion,
+// m = parentMatrix * node.matrix (examples/gltfReader/main.cpp:1538-1576), and
+// transforms a BoundingSphere by a mat4 on the CPU (src/CadR/BoundingSphere.h:70-87).
+// Here a tree is stored level by level (all nodes of depth d contiguous, each
+// with the uint32 index of its parent in level d-1), so one launch per level
+// computes every node of that level in parallel; levels run in order on one
+// stream.  The leaf level can also write the drawables' world spheres, so
+// the sphere transform costs no extra pass over the world matrices.
+//
+// Memory: per node 64 B local read + 64 B world write (+4 B parent index);
+// parents are few and shared by many children, so their reads hit L2.  The
+// matrices are stored as the reference stores them (column-major glm::mat4,
+// 64 B), which keeps the world array usable as MatrixList payload; lane i
+// reads its 64 B as four 16-byte loads, and the four loads of a wave cover
+// one contiguous 4 KiB span.
+#include "internal.h"
+#include "fp32.h"
+
+namespace cadr {
+
+// World sphere of a node: the mesh's local sphere transformed by W, or the
+// empty sphere of BoundingSphere::empty() (BoundingSphere.h:29-31, radius
+// -inf, never visible) for a mesh index outside the table.
+__device__ __forceinline__ float4 nodeSphere(const float4 W[4], const cadrMesh* __restrict__ meshes,
+                                             uint32_t numMeshes, uint32_t m)
+{
+	if (m >= numMeshes) return make_float4(0.f, 0.f, 0.f, -__builtin_inff());
+	const float4 local = *reinterpret_cast<const float4*>(meshes + m);  // center.xyz, radius
+	return sphereTransform(W, local);
+}
+
+template <bool ROOT, bool SPHERES>
+__global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict__ local,
+                                                       const uint32_t* __restrict__ parentIndex,
+                                                       const float4* __restrict__ parentWorld,
+                                                       uint32_t parentCount, float4* __restrict__ world,
+                                                       uint64_t count,
+                                                       const uint32_t* __restrict__ meshIndex,
+                                                       const cadrMesh* __restrict__ meshes, uint32_t numMeshes,
+                                                       float4* __restrict__ spheres)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+	if (i >= count) return;
+	float4 L[4];
+#pragma unroll
+	for (int c = 0; c < 4; c++) L[c] = local[4 * i + c];
+	float4 W[4];
+	if constexpr (ROOT) {
+#pragma unroll
+		for (int c = 0; c < 4; c++) W[c] = L[c];
+	} else {
+		const uint32_t p = parentIndex[i];
+		if (p < parentCount) {
+			float4 P[4];
+#pragma unroll
+			for (int c = 0; c < 4; c++) P[c] = parentWorld[4 * uint64_t(p) + c];
+			mat4Mul(P, L, W);
+		} else {
+#pragma unroll
+			for (int c = 0; c < 4; c++) W[c] = L[c];
+		}
+	}
+#pragma unroll
+	for (int c = 0; c < 4; c++) world[4 * i + c] = W[c];
+	if constexpr (SPHERES) spheres[i] = nodeSphere(W, meshes, numMeshes, meshIndex[i]);
+}
+
+__global__ __launch_bounds__(256) void worldSpheresKernel(const float4* __restrict__ world,
+                                                          const uint32_t* __restrict__ meshIndex,
+                                                          const cadrMesh* __restrict__ meshes, uint32_t numMeshes,
+                                                          uint64_t count, float4* __restrict__ spheres)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+	if (i >= count) return;
+	float4 W[4];
+#pragma unroll
+	for (int c = 0; c < 4; c++) W[c] = world[4 * i + c];
+	spheres[i] = nodeSphere(W, meshes, numMeshes, meshIndex[i]);
+}
+
+}  // namespace cadr
+
+using namespace cadr;
+
+CADR_API int cadrPropagateTransforms(const cadrTransformLevel* lv, cadrStream stream)
+{
+	CADR_REQUIRE(lv, "cadrPropagateTransforms: level is NULL");
+	if (lv->count == 0) return ok();
+	CADR_REQUIRE(lv->localMatrices && lv->worldMatrices, "cadrPropagateTransforms: null matrix array");
+	CADR_REQUIRE((lv->localMatrices % 16) == 0 && (lv->worldMatrices % 16) == 0 && (lv->parentWorld % 16) == 0,
+	             "cadrPropagateTransforms: matrix arrays must be 16-byte aligned");
+	const bool root = lv->parentWorld == 0;
+	CADR_REQUIRE(root || lv->parentIndex, "cadrPropagateTransforms: parentIndex is NULL for a non-root level");
+	const bool spheres = lv->spheres != 0;
+	CADR_REQUIRE(!spheres || (lv->meshIndex && lv->meshes && (lv->spheres % 16) == 0 && (lv->meshes % 16) == 0),
+	             "cadrPropagateTransforms: spheres need meshIndex and 16-byte aligned meshes/spheres");
+	const uint64_t blocks = (lv->count + 255) / 256;
+	CADR_REQUIRE(blocks < (1ull << 31), "cadrPropagateTransforms: level too large");
+	auto* local = reinterpret_cast<const float4*>(lv->localMatrices);
+	auto* parent = reinterpret_cast<const uint32_t*>(lv->parentIndex);
+	auto* pworld = reinterpret_cast<const float4*>(lv->parentWorld);
+	auto* world = reinterpret_cast<float4*>(lv->worldMatrices);
+	auto* mi = reinterpret_cast<const uint32_t*>(lv->meshIndex);
+	auto* meshes = reinterpret_cast<const cadrMesh*>(lv->meshes);
+	auto* sph = reinterpret_cast<float4*>(lv->spheres);
+	hipStream_t s = hipStream_t(stream);
+	const dim3 g{uint32_t(blocks)}, b{256};
+	if (root && spheres) hipLaunchKernelGGL((propagateKernel<true, true>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
+	else if (root) hipLaunchKernelGGL((propagateKernel<true, false>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
+	else if (spheres) hipLaunchKernelGGL((propagateKernel<false, true>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
+	else hipLaunchKernelGGL((propagateKernel<false, false>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
+	CADR_CHECK_HIP(hipGetLastError(), "propagateKernel launch");
+	return ok();
+}
+
+CADR_API int cadrComputeWorldSpheres(uint64_t worldMatrices, uint64_t meshIndex, uint64_t meshes,
+                                     uint32_t numMeshes, uint64_t count, uint64_t spheres, cadrStream stream)
+{
+	if (count == 0) return ok();
+	CADR_REQUIRE(worldMatrices && meshIndex && meshes && spheres, "cadrComputeWorldSpheres: null address");
+	CADR_REQUIRE((worldMatrices % 16) == 0 && (meshes % 16) == 0 && (spheres % 16) == 0,
+	             "cadrComputeWorldSpheres: arrays must be 16-byte aligned");
+	const uint64_t blocks = (count + 255) / 256;
+	CADR_REQUIRE(blocks < (1ull << 31), "cadrComputeWorldSpheres: count too large");
+	hipLaunchKernelGGL(worldSpheresKernel, dim3(uint32_t(blocks)), dim3(256), 0, hipStream_t(stream),
+	                   reinterpret_cast<const float4*>(worldMatrices), reinterpret_cast<const uint32_t*>(meshIndex),
+	                   reinterpret_cast<const cadrMesh*>(meshes), numMeshes, count, reinterpret_cast<float4*>(spheres));
+	CADR_CHECK_HIP(hipGetLastError(), "worldSpheresKernel launch");
+	return ok();
+}

@@ -1,0 +1,295 @@
+"""Deterministic synthetic scenes for the drawable-processing pass.
+
+Counter-based SplitMix64 (every value is a pure function of seed, stream,
+item and component, so any slice can be generated independently); floats
+take the top 24 bits.  Seed = 0xCAD00000 + workload index.  The workloads:
+
+  plain100k  100,000 drawables, 64 meshes (indexCount U{36..3000}, firstIndex
+             cumulative, radius U[0.5, 2.5)), flat transforms: translation
+             U[-500,500]^3, uniform scale exp(U[ln 0.1, ln 2]); 1 StateSet;
+             camera at the origin looking -z, vfov 60 deg, 16:9, near 0.1,
+             far 2000, 1080-pixel viewport.
+  full10m    transform tree 1,000 roots -> 100 children -> 100 leaves
+             (10,101,000 nodes, 10,000,000 leaf drawables); local matrices are
+             a uniform random rotation (Shoemake quaternion) plus translation
+             U[-50,50]^3; 64 meshes with 3 LODs (LOD k index count roughly
+             count/4^k), LOD thresholds 64 px and 8 px at 1080 px; StateSet of
+             each leaf U{0..63}, drawables grouped by StateSet in tree order.
+  batch256   the full10m scene with 256 views: eye U[-600,600]^3, uniform
+             random direction, vfov U[40, 90] deg, 16:9.
+  uneven64   64 scenes of round(exp(U[ln 1e4, ln 5e6])) drawables, tree depth
+             U{1..4} (equal fan-out per level), LOD count U{1..4}, 64
+             StateSets, 1..32 views each.
+  served128  128 flat scenes of round(exp(U[ln 1e4, ln 2e7])) drawables
+             (plain100k structure, 64 StateSets), trimmed to a byte budget.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._native import MESH_DTYPE
+
+GAMMA = np.uint64(0x9E3779B97F4A7C15)
+M1 = np.uint64(0xBF58476D1CE4E5B9)
+M2 = np.uint64(0x94D049BB133111EB)
+SEED_BASE = 0xCAD00000
+
+WORKLOADS = ("plain100k", "full10m", "batch256", "uneven64", "served128")
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * M1
+    z = (z ^ (z >> np.uint64(27))) * M2
+    return z ^ (z >> np.uint64(31))
+
+
+def rand_u64(seed: int, stream: int, start: int, count: int, comps: int = 1) -> np.ndarray:
+    """SplitMix64 outputs for items start..start+count-1, comps values each
+    (shape (count, comps)).  Counter = stream<<40 | item*comps + comp."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(start * comps, (start + count) * comps, dtype=np.uint64)
+        ctr = (np.uint64(stream) << np.uint64(40)) + idx + np.uint64(1)
+        x = np.uint64(seed) + ctr * GAMMA
+        return _mix(x).reshape(count, comps)
+
+
+def rand_unit(seed: int, stream: int, start: int, count: int, comps: int = 1) -> np.ndarray:
+    """Uniform floats in [0, 1) from the top 24 bits (float32)."""
+    u = rand_u64(seed, stream, start, count, comps) >> np.uint64(40)
+    return (u.astype(np.float32) * np.float32(1.0 / 16777216.0)).astype(np.float32)
+
+
+def rand_int(seed: int, stream: int, start: int, count: int, lo: int, hi: int) -> np.ndarray:
+    """Uniform integers in [lo, hi] (inclusive) from the top 32 bits."""
+    u = rand_u64(seed, stream, start, count, 1)[:, 0] >> np.uint64(32)
+    span = np.uint64(hi - lo + 1)
+    return (lo + ((u * span) >> np.uint64(32)).astype(np.int64)).astype(np.int64)
+
+
+# ------------------------------------------------------------------ meshes
+
+def make_meshes(seed: int, count: int, lod_counts) -> np.ndarray:
+    """Mesh table: centre 0, radius U[0.5, 2.5), LOD k index count derived
+    from a base count U{36..3000} as max(12, (c0 >> 2k) rounded down to a
+    multiple of 3), firstIndex cumulative over (mesh, LOD)."""
+    lod_counts = np.broadcast_to(np.asarray(lod_counts, dtype=np.int64), (count,))
+    base = rand_int(seed, 1, 0, count, 36, 3000)
+    radius = np.float32(0.5) + np.float32(2.0) * rand_unit(seed, 2, 0, count)[:, 0]
+    m = np.zeros(count, dtype=MESH_DTYPE)
+    m["radius"] = radius
+    m["lodCount"] = lod_counts
+    first = 0
+    for i in range(count):
+        for k in range(int(lod_counts[i])):
+            c = int(base[i]) >> (2 * k)
+            c = max(12, c - c % 3)
+            m["indexCount"][i, k] = c
+            m["firstIndex"][i, k] = first
+            first += c
+    return m
+
+
+# --------------------------------------------------------------- matrices
+
+def rotation_translation(seed: int, stream: int, start: int, count: int, trange: float) -> np.ndarray:
+    """count x 16 float32 column-major matrices: uniform random rotation
+    (Shoemake's quaternion from three uniforms) and translation U[-t, t]^3."""
+    u = rand_unit(seed, stream, start, count, 6).astype(np.float64)
+    u1, u2, u3 = u[:, 0], u[:, 1], u[:, 2]
+    a = np.sqrt(1.0 - u1)
+    b = np.sqrt(u1)
+    x = a * np.sin(2 * math.pi * u2)
+    y = a * np.cos(2 * math.pi * u2)
+    z = b * np.sin(2 * math.pi * u3)
+    w = b * np.cos(2 * math.pi * u3)
+    m = np.zeros((count, 16), dtype=np.float32)
+    # column 0
+    m[:, 0] = 1 - 2 * (y * y + z * z)
+    m[:, 1] = 2 * (x * y + z * w)
+    m[:, 2] = 2 * (x * z - y * w)
+    # column 1
+    m[:, 4] = 2 * (x * y - z * w)
+    m[:, 5] = 1 - 2 * (x * x + z * z)
+    m[:, 6] = 2 * (y * z + x * w)
+    # column 2
+    m[:, 8] = 2 * (x * z + y * w)
+    m[:, 9] = 2 * (y * z - x * w)
+    m[:, 10] = 1 - 2 * (x * x + y * y)
+    # column 3: translation
+    t = u[:, 3:6].astype(np.float32)
+    m[:, 12:15] = (np.float32(-trange) + np.float32(2 * trange) * t).astype(np.float32)
+    m[:, 15] = 1.0
+    return m
+
+
+def scale_translation(seed: int, stream: int, start: int, count: int) -> np.ndarray:
+    """Flat transforms of plain100k: scale exp(U[ln 0.1, ln 2]), translation U[-500,500]^3."""
+    u = rand_unit(seed, stream, start, count, 4)
+    lo, hi = math.log(0.1), math.log(2.0)
+    s = np.exp(lo + u[:, 0].astype(np.float64) * (hi - lo)).astype(np.float32)
+    m = np.zeros((count, 16), dtype=np.float32)
+    m[:, 0] = s
+    m[:, 5] = s
+    m[:, 10] = s
+    m[:, 12:15] = (np.float32(-500.0) + np.float32(1000.0) * u[:, 1:4]).astype(np.float32)
+    m[:, 15] = 1.0
+    return m
+
+
+# ------------------------------------------------------------------ views
+
+@dataclass
+class Camera:
+    eye: tuple
+    dir: tuple
+    up: tuple
+    fovy: float            # radians
+    aspect: float = 16.0 / 9.0
+    near: float = 0.1
+    far: float = 2000.0
+    viewport_height: float = 1080.0
+    thresholds: tuple = (64.0, 8.0, 1.0)
+
+
+DEFAULT_CAMERA = Camera(eye=(0.0, 0.0, 0.0), dir=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0),
+                        fovy=float(np.float32(math.radians(60.0))))
+
+
+def random_cameras(seed: int, stream: int, start: int, count: int, box: float = 600.0) -> list:
+    u = rand_unit(seed, stream, start, count, 6).astype(np.float64)
+    cams = []
+    for i in range(count):
+        eye = tuple(float(np.float32(-box + 2 * box * u[i, k])) for k in range(3))
+        z = 2.0 * u[i, 3] - 1.0
+        phi = 2.0 * math.pi * u[i, 4]
+        r = math.sqrt(max(0.0, 1.0 - z * z))
+        d = (float(np.float32(r * math.cos(phi))), float(np.float32(r * math.sin(phi))), float(np.float32(z)))
+        up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.99 else (1.0, 0.0, 0.0)
+        fovy = float(np.float32(math.radians(40.0 + 50.0 * u[i, 5])))
+        cams.append(Camera(eye=eye, dir=d, up=up, fovy=fovy))
+    return cams
+
+
+# ------------------------------------------------------------------ scenes
+
+@dataclass
+class Scene:
+    """A scene in the pass's input layout.
+
+    levels[d] = (local float32[n_d,16], parent uint32[n_d] or None); the last
+    level holds the drawables (leaves) in drawable order.  mesh_index and
+    stateset_counts describe the drawables; meshes is the mesh table.
+    """
+    name: str
+    levels: list
+    mesh_index: np.ndarray
+    meshes: np.ndarray
+    stateset_counts: np.ndarray
+    cameras: list = field(default_factory=list)
+
+    @property
+    def num_drawables(self) -> int:
+        return int(self.mesh_index.shape[0])
+
+
+def _group_by_stateset(seed, stream, leaf_count, num_statesets, start=0):
+    ss = rand_int(seed, stream, start, leaf_count, 0, num_statesets - 1)
+    order = np.argsort(ss, kind="stable")
+    counts = np.bincount(ss, minlength=num_statesets).astype(np.uint32)
+    return order, counts
+
+
+def plain_scene(n: int = 100_000, seed: int = SEED_BASE + 0, num_statesets: int = 1,
+                num_meshes: int = 64) -> Scene:
+    meshes = make_meshes(seed, num_meshes, 1)
+    mesh_index = rand_int(seed, 3, 0, n, 0, num_meshes - 1).astype(np.uint32)
+    world = scale_translation(seed, 4, 0, n)
+    if num_statesets > 1:
+        order, counts = _group_by_stateset(seed, 5, n, num_statesets)
+        world = world[order]
+        mesh_index = mesh_index[order]
+    else:
+        counts = np.array([n], dtype=np.uint32)
+    return Scene("plain", [(np.ascontiguousarray(world), None)], np.ascontiguousarray(mesh_index), meshes,
+                 counts, [DEFAULT_CAMERA])
+
+
+def tree_scene(fanouts=(1000, 100, 100), seed: int = SEED_BASE + 1, num_statesets: int = 64,
+               lod_count: int = 3, num_meshes: int = 64, trange: float = 50.0) -> Scene:
+    """Tree with level sizes fanouts[0], fanouts[0]*fanouts[1], ...; the leaf
+    level is reordered into StateSet groups (tree order inside a group)."""
+    meshes = make_meshes(seed, num_meshes, lod_count)
+    levels = []
+    n_prev = None
+    n = 1
+    for d, f in enumerate(fanouts):
+        n = n * f
+        local = rotation_translation(seed, 10 + d, 0, n, trange)
+        parent = None if d == 0 else (np.arange(n, dtype=np.uint64) // f).astype(np.uint32)
+        levels.append([local, parent])
+        n_prev = n
+    leaves = n_prev
+    mesh_index = rand_int(seed, 3, 0, leaves, 0, num_meshes - 1).astype(np.uint32)
+    order, counts = _group_by_stateset(seed, 5, leaves, num_statesets)
+    local, parent = levels[-1]
+    levels[-1] = [np.ascontiguousarray(local[order]), None if parent is None else np.ascontiguousarray(parent[order])]
+    mesh_index = np.ascontiguousarray(mesh_index[order])
+    return Scene("tree", [tuple(l) for l in levels], mesh_index, meshes, counts, [DEFAULT_CAMERA])
+
+
+def full10m_scene(scale: float = 1.0) -> Scene:
+    """The full10m workload; scale < 1 shrinks the root count (for tests)."""
+    roots = max(1, int(round(1000 * scale)))
+    return tree_scene((roots, 100, 100), seed=SEED_BASE + 1)
+
+
+def batch_cameras(count: int = 256) -> list:
+    return random_cameras(SEED_BASE + 2, 20, 0, count)
+
+
+def uneven_specs(num_scenes: int = 64, lo: float = 1e4, hi: float = 5e6, max_views: int = 32,
+                 seed: int = SEED_BASE + 3):
+    """(drawables, depth, lod_count, views) per scene of the uneven64 workload."""
+    u = rand_unit(seed, 30, 0, num_scenes, 1)[:, 0].astype(np.float64)
+    sizes = np.rint(np.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))).astype(np.int64)
+    depth = rand_int(seed, 31, 0, num_scenes, 1, 4)
+    lods = rand_int(seed, 32, 0, num_scenes, 1, 4)
+    views = rand_int(seed, 33, 0, num_scenes, 1, max_views)
+    return [(int(sizes[i]), int(depth[i]), int(lods[i]), int(views[i])) for i in range(num_scenes)]
+
+
+def uneven_scene(index: int, drawables: int, depth: int, lod_count: int, num_views: int,
+                 seed: int = SEED_BASE + 3, num_statesets: int = 64) -> Scene:
+    """Scene `index` of uneven64: `depth` levels with equal fan-out
+    f = ceil(drawables^(1/depth)); level d has ceil(n_{d+1}/f) nodes."""
+    sseed = seed + 0x1000 * (index + 1)
+    f = max(1, int(math.ceil(drawables ** (1.0 / depth) - 1e-9)))
+    sizes = [drawables]
+    for _ in range(depth - 1):
+        sizes.append((sizes[-1] + f - 1) // f)
+    sizes = sizes[::-1]
+    meshes = make_meshes(sseed, 64, lod_count)
+    levels = []
+    for d, n in enumerate(sizes):
+        local = rotation_translation(sseed, 10 + d, 0, n, 50.0)
+        parent = None if d == 0 else (np.arange(n, dtype=np.uint64) // f).astype(np.uint32)
+        levels.append([local, parent])
+    mesh_index = rand_int(sseed, 3, 0, drawables, 0, 63).astype(np.uint32)
+    order, counts = _group_by_stateset(sseed, 5, drawables, num_statesets)
+    local, parent = levels[-1]
+    levels[-1] = [np.ascontiguousarray(local[order]), None if parent is None else np.ascontiguousarray(parent[order])]
+    cams = random_cameras(sseed, 20, 0, num_views)
+    return Scene(f"uneven{index}", [tuple(l) for l in levels], np.ascontiguousarray(mesh_index[order]), meshes,
+                 counts, cams)
+
+
+def served_specs(num_scenes: int = 128, lo: float = 1e4, hi: float = 2e7, budget_drawables: int | None = None,
+                 seed: int = SEED_BASE + 4):
+    u = rand_unit(seed, 40, 0, num_scenes, 1)[:, 0].astype(np.float64)
+    sizes = np.rint(np.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))).astype(np.int64)
+    if budget_drawables is not None and sizes.sum() > budget_drawables:
+        sizes = np.maximum(1, (sizes * (budget_drawables / sizes.sum())).astype(np.int64))
+    return [int(s) for s in sizes]

@@ -1,0 +1,343 @@
+/*
+ * cadr_hip.h — C ABI of the MI355X (gfx950) drawable-processing pass.
+ *
+ * This is the drop-in boundary of libcadr_hip.so.  Every entry point takes
+ * plain integers, device addresses (uint64_t, as Vulkan buffer device
+ * addresses are in the reference) and sizes; nothing here uses C++ or torch
+ * types.  Errors are returned as cadrResult codes, never thrown.  The detail
+ * text of the last error of the calling thread is kept by
+ * cadrGetLastErrorMessage().
+ *
+ * Reference interfaces replaced (paths relative to the Rendering-FIT/CADR
+ * tree):
+ *   - the compute pass   src/CadR/shaders/processDrawables.comp:92-113
+ *     and its recording  src/CadR/Renderer.cpp:598-720
+ *     (Renderer::recordDrawableProcessing, push constants :671-683,
+ *     32768-split dispatch :684-692);
+ *   - the CPU-side hierarchy propagation examples/gltfReader/main.cpp:1538-1576
+ *     (m = parentMatrix * node.matrix, glm mat4*mat4
+ *     3rdParty/GLM/glm/detail/type_mat4x4.inl:630-648);
+ *   - the world bounding sphere src/CadR/BoundingSphere.h:70-87;
+ *   - the per-StateSet draw ranges src/CadR/StateSet.cpp:198-268
+ *     (StateSet k owns commands [base_k, base_k + n_k) of the indirect buffer).
+ * Frustum culling, LOD selection and compaction have no reference
+ * implementation; their definition is given with cadrCullView below.
+ *
+ * Thread safety: like the reference (src/CadR has no locks), calls that share
+ * a stream or a cull workspace must be serialised by the caller.
+ */
+#ifndef CADR_HIP_H
+#define CADR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Results.  The reference throws CadR::LogicError / OutOfResources / Timeout
+ * (src/CadR/Exceptions.h:13-40) and vk::SystemError; across the C ABI these
+ * become the codes below. */
+typedef enum cadrResult {
+	CADR_SUCCESS = 0,
+	CADR_ERROR_INVALID_VALUE = 1,     /* LogicError: bad argument or layout */
+	CADR_ERROR_OUT_OF_RESOURCES = 2,  /* OutOfResources: allocation failed  */
+	CADR_ERROR_TIMEOUT = 3,           /* Timeout: a wait did not finish     */
+	CADR_ERROR_DEVICE = 4,            /* HIP runtime error (vk::SystemError) */
+	CADR_ERROR_NOT_READY = 5          /* query: work not finished yet       */
+} cadrResult;
+
+const char* cadrGetResultString(int result);
+const char* cadrGetLastErrorMessage(void);
+
+/* HIP stream handle (hipStream_t); 0 means the legacy default stream. */
+typedef void* cadrStream;
+/* HIP event handle (hipEvent_t). */
+typedef void* cadrEvent;
+
+/* ------------------------------------------------------------------------ */
+/* Device, memory and streams.  Stand-ins for the Vulkan device, DataMemory
+ * blocks and the staging path (src/CadR/Renderer.cpp:97-313, 946-999,
+ * src/CadR/DataMemory.cpp:255-397). */
+int cadrSetDevice(int deviceIndex);
+int cadrGetDeviceCount(int* count);
+/* Number of compute units of the current device (256 on MI355X). */
+int cadrGetComputeUnitCount(int* count);
+int cadrDeviceMalloc(uint64_t numBytes, uint64_t* deviceAddress);
+int cadrDeviceFree(uint64_t deviceAddress);
+int cadrHostMallocPinned(uint64_t numBytes, void** hostPointer);
+int cadrHostFreePinned(void* hostPointer);
+int cadrMemcpyHtoD(uint64_t dst, const void* src, uint64_t numBytes, cadrStream stream);
+int cadrMemcpyDtoH(void* dst, uint64_t src, uint64_t numBytes, cadrStream stream);
+int cadrMemcpyDtoD(uint64_t dst, uint64_t src, uint64_t numBytes, cadrStream stream);
+/* Fills numBytes/4 uint32 words with value using a kernel (capturable in a
+ * HIP graph, unlike hipMemsetAsync). numBytes must be a multiple of 4. */
+int cadrFillU32(uint64_t dst, uint32_t value, uint64_t numBytes, cadrStream stream);
+int cadrStreamCreate(cadrStream* stream);
+int cadrStreamDestroy(cadrStream stream);
+int cadrStreamSynchronize(cadrStream stream);
+int cadrDeviceSynchronize(void);
+int cadrEventCreate(cadrEvent* event);
+int cadrEventDestroy(cadrEvent event);
+int cadrEventRecord(cadrEvent event, cadrStream stream);
+int cadrEventSynchronize(cadrEvent event);
+/* CADR_SUCCESS when complete, CADR_ERROR_NOT_READY otherwise. */
+int cadrEventQuery(cadrEvent event);
+int cadrEventElapsedTime(float* milliseconds, cadrEvent start, cadrEvent stop);
+
+/* ------------------------------------------------------------------------ */
+/* Byte layouts shared with the reference (all little endian).              */
+
+/* DrawableGpuData, 48 B: src/CadR/Drawable.h:32-43, static_assert
+ * src/CadR/Drawable.cpp:13, GLSL view processDrawables.comp:17-26. */
+typedef struct cadrDrawableGpuData {
+	uint64_t vertexDataHandle;
+	uint64_t indexDataHandle;
+	uint64_t matrixListHandle;
+	uint64_t drawableDataHandle;
+	uint64_t primitiveSetHandle;
+	uint32_t primitiveSetOffset;   /* byte offset added to the PrimitiveSet address */
+	uint32_t padding;
+} cadrDrawableGpuData;
+
+/* PrimitiveSet GPU record, 8 B: processDrawables.comp:29-33,
+ * src/CadR/PrimitiveSet.h:12-15. */
+typedef struct cadrPrimitiveSet {
+	uint32_t count;
+	uint32_t first;
+} cadrPrimitiveSet;
+
+/* MatrixList GPU header, 64 B, followed by column-major float mat4[]:
+ * processDrawables.comp:35-41, src/CadR/MatrixList.h:54-59. */
+typedef struct cadrMatrixListHeader {
+	uint32_t numMatrices;
+	uint32_t capacity;
+	uint64_t padding[7];
+} cadrMatrixListHeader;
+
+/* VkDrawIndirectCommand, 16 B: processDrawables.comp:43-50. */
+typedef struct cadrDrawIndirectCommand {
+	uint32_t vertexCount;
+	uint32_t instanceCount;
+	uint32_t firstVertex;
+	uint32_t firstInstance;
+} cadrDrawIndirectCommand;
+
+/* DrawablePointers, 32 B: processDrawables.comp:52-59,
+ * Renderer::drawablePointersRecordSize src/CadR/Renderer.h:201. */
+typedef struct cadrDrawablePointers {
+	uint64_t vertexDataPtr;
+	uint64_t indexDataPtr;
+	uint64_t matrixListPtr;
+	uint64_t drawableDataPtr;
+} cadrDrawablePointers;
+
+/* VkDrawIndexedIndirectCommand, 20 B: the compacted output of the cull pass. */
+typedef struct cadrDrawIndexedIndirectCommand {
+	uint32_t indexCount;
+	uint32_t instanceCount;
+	uint32_t firstIndex;
+	int32_t  vertexOffset;
+	uint32_t firstInstance;
+} cadrDrawIndexedIndirectCommand;
+
+/* Push-constant block of processDrawables.comp:68-74, filled by
+ * Renderer::recordDrawableProcessing (src/CadR/Renderer.cpp:671-683). */
+typedef struct cadrProcessDrawablesPushConstants {
+	uint64_t handleTable;        /* root table device address (DataStorage::handleTableDeviceAddress) */
+	uint64_t drawableList;       /* cadrDrawableGpuData[numDrawables], 16-B aligned */
+	uint64_t indirectData;       /* cadrDrawIndirectCommand[numDrawables], 16-B aligned */
+	uint64_t drawablePointers;   /* cadrDrawablePointers[numDrawables], 16-B aligned */
+} cadrProcessDrawablesPushConstants;
+
+/* Handle table geometry: src/CadR/HandleTable.h:33-35. */
+#define CADR_HANDLES_PER_TABLE 2048u
+#define CADR_HANDLE_LEVEL_SHIFT 11u
+#define CADR_HANDLE_LEVEL_MASK 0x7ffu
+/* Largest drawable count one call accepts: the reference asserts
+ * numDrawables < 32768*32768 (src/CadR/Renderer.cpp:687). */
+#define CADR_MAX_DRAWABLES (32768ull * 32768ull - 1ull)
+
+/* ------------------------------------------------------------------------ */
+/* Gather pass: processDrawables.comp:92-113, dispatched as
+ * Renderer::recordDrawableProcessing does (src/CadR/Renderer.cpp:668-692).
+ *
+ * For every drawable i < numDrawables:
+ *   d   = drawableList[i]
+ *   ps  = lookup(d.primitiveSetHandle) + d.primitiveSetOffset
+ *   ml  = lookup(d.matrixListHandle)
+ *   indirectData[i]     = { ps.count, ml.numMatrices, ps.first, 0 }
+ *   drawablePointers[i] = { lookup(vertex), lookup(index), ml, lookup(drawableData) }
+ * with lookup() walking handleLevel (1, 2 or 3) tables of 2048 uint64
+ * entries (processDrawables.comp:77-89).
+ * Invalid handles whose table walk leaves the tables that exist resolve to
+ * address 0 (the reference reads unowned memory there), and a PrimitiveSet
+ * or MatrixList that resolves to 0 yields zero fields instead of a read
+ * through a null address.  numDrawables == 0 is a no-op, as in the
+ * reference (Renderer.cpp:600-620). */
+int cadrProcessDrawables(const cadrProcessDrawablesPushConstants* pushConstants,
+                         uint32_t handleLevel, uint64_t numDrawables, cadrStream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Transform propagation (one tree level per call) and world bounding
+ * spheres.  Matrices are column-major float mat4 (64 B), as glm::mat4 and
+ * the MatrixList payload.
+ *
+ *   world[i] = parentWorld[parentIndex[i]] * local[i]    (glm operator*)
+ * evaluated column by column as
+ *   world[i][j] = ((P[0]*L[j][0] + P[1]*L[j][1]) + P[2]*L[j][2]) + P[3]*L[j][3]
+ * in float with no fused multiply-add (type_mat4x4.inl:643-646).  A level
+ * with parentWorld == 0 is a root level: world = local, copied bit for bit.
+ *
+ * If spheres != 0, the level also writes the world bounding sphere of each
+ * node (BoundingSphere.h:70-87) from the mesh of meshIndex[i]:
+ *   c' = mat3(W)*c + W[3].xyz     (mat3*vec3 left to right, type_mat3x3.inl:470-473)
+ *   r' = sqrt(max(max(|W0|^2, |W1|^2), |W2|^2)) * r,  |v|^2 = (x*x + y*y) + z*z
+ * as float4 {x, y, z, r}. */
+typedef struct cadrMesh {
+	float    center[3];         /* local bounding sphere                     */
+	float    radius;
+	uint32_t lodCount;          /* 1..4; LOD k is PrimitiveSet k of the mesh */
+	int32_t  vertexOffset;      /* copied into the indexed draw command      */
+	uint32_t reserved[2];
+	uint32_t indexCount[4];     /* per LOD {count, first} (PrimitiveSet.h)   */
+	uint32_t firstIndex[4];
+} cadrMesh;                     /* 64 B */
+
+typedef struct cadrTransformLevel {
+	uint64_t localMatrices;     /* float mat4[count]                          */
+	uint64_t parentIndex;       /* uint32[count]; ignored for a root level     */
+	uint64_t parentWorld;       /* float mat4[] of the previous level, or 0    */
+	uint64_t worldMatrices;     /* float mat4[count] output                    */
+	uint64_t count;
+	uint64_t meshIndex;         /* uint32[count]; used when spheres != 0       */
+	uint64_t meshes;            /* cadrMesh[]                                  */
+	uint64_t spheres;           /* float4[count] output, or 0                  */
+	uint32_t numMeshes;         /* meshIndex >= numMeshes gives BoundingSphere::empty(), radius -inf */
+	uint32_t parentCount;       /* nodes in the parent level; parentIndex >= parentCount acts as a root */
+} cadrTransformLevel;
+
+int cadrPropagateTransforms(const cadrTransformLevel* level, cadrStream stream);
+
+/* World spheres of drawables whose world matrices are already known (flat
+ * scenes): spheres[i] = worldMatrices[i] * meshes[meshIndex[i]].sphere. */
+int cadrComputeWorldSpheres(uint64_t worldMatrices, uint64_t meshIndex, uint64_t meshes,
+                            uint32_t numMeshes, uint64_t count, uint64_t spheres, cadrStream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Frustum cull, LOD selection and per-StateSet compaction.
+ *
+ * Scene: numDrawables drawables grouped by StateSet in the order the
+ * reference assigns draw ranges (StateSet DFS, StateSet.cpp:198-268):
+ * StateSet k owns drawables [base_k, base_k + n_k), base_k = n_0 + ... + n_{k-1}.
+ * Per drawable: a world sphere float4 {x,y,z,r} and a uint32 mesh index.
+ *
+ * View: a world-to-view matrix in glm lookAtLH convention (+z forward) and
+ * frustum constants, all precomputed on the host (cadrMakeView).  For each
+ * drawable with sphere (c, r) and view V, in float without FMA:
+ *   v.x = (V[0][0]*c.x + V[1][0]*c.y) + (V[2][0]*c.z + V[3][0])   (glm mat4*vec4
+ *   v.y, v.z likewise                                                order, :561-572)
+ *   visible = (v.z + r >= nearZ) && (v.z - r <= farZ)
+ *          && (|v.x|*sideX[0] - v.z*sideX[1] <= r)
+ *          && (|v.y|*sideY[0] - v.z*sideY[1] <= r)
+ *   p   = r * lodScale                         (projected radius = p / v.z px)
+ *   lod = (p < lodThreshold[0]*v.z) + (p < lodThreshold[1]*v.z) + (p < lodThreshold[2]*v.z)
+ *   lod = min(lod, mesh.lodCount - 1)
+ * Output for view w: commands_w[base_k + j], j = 0 .. count_k - 1, are the
+ * visible drawables of StateSet k in increasing drawable order, each
+ *   { mesh.indexCount[lod], 1, mesh.firstIndex[lod], mesh.vertexOffset, i }
+ * (firstInstance = drawable index i), and counts_w[k] = count_k.
+ * Command slots past count_k are left unwritten. */
+typedef struct cadrView {
+	float viewRow[3][4];        /* rows 0..2 of the world->view mat4: {V[0][r], V[1][r], V[2][r], V[3][r]} */
+	float nearZ;
+	float farZ;
+	float sideX[2];             /* {cos, sin} of the horizontal half angle */
+	float sideY[2];             /* {cos, sin} of the vertical half angle   */
+	float lodScale;             /* viewportHeight/2 * P[1][1] of perspectiveLH_ZO */
+	float lodThreshold[3];      /* projected-radius px thresholds, descending; +inf = unused */
+	float reserved[2];
+} cadrView;                     /* 96 B, 16-byte aligned in device arrays */
+
+/* Builds a cadrView for a camera at eye looking along dir with the given up
+ * vector, vertical field of view fovy (radians), aspect = width/height,
+ * depth range [zNear, zFar] and viewport height in pixels.  The view
+ * matrix follows glm::lookAtLH (ext/matrix_transform.inl), P[1][1] =
+ * 1/tan(fovy/2) follows glm::perspectiveLH_ZO (ext/matrix_clip_space.inl:255-268).
+ * lodThresholds (3 floats, descending) may be NULL for {64, 8, 1} px. */
+int cadrMakeView(const float eye[3], const float dir[3], const float up[3], float fovy,
+                 float aspect, float zNear, float zFar, float viewportHeight,
+                 const float* lodThresholds, cadrView* out);
+
+/* A cull tile: a run of at most tileSize drawables inside one StateSet. */
+typedef struct cadrCullTile {
+	uint32_t start;             /* first drawable                                  */
+	uint32_t count;             /* 0 only for the single tile of an empty StateSet */
+	uint32_t stateSetBase;      /* base_k of the tile's StateSet                   */
+	uint32_t stateSetAndFlags;  /* k | CADR_TILE_FIRST | CADR_TILE_LAST            */
+} cadrCullTile;
+#define CADR_TILE_FIRST 0x80000000u
+#define CADR_TILE_LAST  0x40000000u
+#define CADR_TILE_STATESET_MASK 0x3fffffffu
+
+/* Splits StateSets of sizes stateSetCounts[0..numStateSets) into tiles of at
+ * most tileSize drawables.  With tiles == NULL it only counts them. */
+int cadrBuildCullTiles(const uint32_t* stateSetCounts, uint32_t numStateSets, uint32_t tileSize,
+                       cadrCullTile* tiles, uint32_t capacity, uint32_t* numTiles);
+
+/* One scene culled for a run of views: the unit of a batched launch. */
+typedef struct cadrCullJob {
+	uint64_t spheres;           /* float4[numDrawables]                        */
+	uint64_t meshIndex;         /* uint32[numDrawables]                        */
+	uint64_t meshes;            /* cadrMesh[]                                  */
+	uint64_t tiles;             /* cadrCullTile[numTiles] built for the launch tile size */
+	uint64_t views;             /* cadrView[numViews]                          */
+	uint64_t commands;          /* cadrDrawIndexedIndirectCommand[numViews][commandStride] */
+	uint64_t counts;            /* uint32[numViews][numStateSets]              */
+	uint32_t numDrawables;
+	uint32_t numTiles;
+	uint32_t numStateSets;
+	uint32_t numViews;
+	uint32_t commandStride;     /* commands per view, >= numDrawables          */
+	uint32_t numMeshes;         /* drawables with meshIndex >= numMeshes are never visible */
+} cadrCullJob;                  /* 80 B */
+
+/* Kernel shape.  tileSize = blockSize * itemsPerThread. */
+typedef struct cadrCullConfig {
+	uint32_t blockSize;         /* 256 or 512 threads                          */
+	uint32_t itemsPerThread;    /* 2, 4, 8 or 16                               */
+	uint32_t workgroupsPerCU;   /* persistent grid = CUs * this (0 = default)  */
+	uint32_t flags;             /* CADR_CULL_NO_TILE_SPHERES: test every drawable */
+} cadrCullConfig;
+#define CADR_CULL_NO_TILE_SPHERES 1u
+
+/* Device workspace of the cull launches: look-back status words, tile
+ * tickets, the launch epoch and per-job offsets.  One workspace serves one
+ * stream at a time; a launch needs statusWords >= sum over its jobs of
+ * numViews * numTiles and numJobs <= maxJobs.  Status words carry the launch
+ * epoch, so nothing is cleared between launches. */
+typedef struct cadrCullWorkspace cadrCullWorkspace;
+int cadrCullWorkspaceCreate(uint64_t statusWords, uint32_t maxJobs, cadrCullWorkspace** workspace);
+int cadrCullWorkspaceDestroy(cadrCullWorkspace* workspace);
+
+/* Culls numJobs jobs (job descriptors in device memory, cadrCullJob[numJobs])
+ * in one persistent launch.  hostJobs is a host copy of the same
+ * descriptors, used for argument checks and sizing; it is not read by the
+ * device.  Capturable in a HIP graph. */
+int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, const cadrCullJob* hostJobs,
+                  uint32_t numJobs, cadrCullWorkspace* workspace, cadrStream stream);
+
+/* One scene, one view: cadrCullBatch of a single job with numViews == 1. */
+int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCullJob* hostJob,
+                 cadrCullWorkspace* workspace, cadrStream stream);
+
+/* Kernel tile size for a config (blockSize * itemsPerThread), 0 if unsupported. */
+uint32_t cadrCullTileSize(const cadrCullConfig* config);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CADR_HIP_H */

@@ -1,0 +1,157 @@
+"""ctypes loader of the CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, always as the checker.  liboracle.so restates the reference's algorithm
+(see cadr_oracle.h); _ref/libcadr_ref.so, when present, is the reference's own
+BoundingSphere.h + GLM compiled from the reference tree (oracle/Makefile).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(_HERE, "build", "liboracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "libcadr_ref.so")
+
+_P = ctypes.c_void_p
+_u32, _u64, _f, _i = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_int
+
+
+class Region(ctypes.Structure):
+    _fields_ = [("base", _u64), ("size", _u64), ("host", _P)]
+
+
+_oracle = None
+_ref = None
+
+
+def lib() -> ctypes.CDLL:
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"{ORACLE_PATH} missing: run make -C oracle")
+        L = ctypes.CDLL(ORACLE_PATH)
+        sig = {
+            "oracle_process_drawables": (_i, [_P, _u32, _P, _u32, _u64]),
+            "oracle_lookup_handle": (_i, [_P, _u32, _u64, _u32, _u64, ctypes.POINTER(_u64)]),
+            "oracle_mat4_mul": (None, [_P, _P, _P]),
+            "oracle_sphere_transform": (None, [_P, _P, _P]),
+            "oracle_propagate_level": (None, [_P, _P, _u32, _P, _P, _u64, _P, _P, _u32, _P]),
+            "oracle_make_view": (_i, [_P, _P, _P, _f, _f, _f, _f, _f, _P, _P]),
+            "oracle_test_drawable": (_i, [_P, _P, _P]),
+            "oracle_cull": (None, [_P, _P, _P, _u32, _P, _u32, _P, _u32, _u32, _P, _P, _i]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _oracle = L
+    return _oracle
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref() -> ctypes.CDLL:
+    global _ref
+    if _ref is None:
+        L = ctypes.CDLL(REF_PATH)
+        sig = {
+            "ref_mat4_mul": (None, [_P, _P, _P]),
+            "ref_mat4_vec4": (None, [_P, _P, _P]),
+            "ref_sphere_transform": (None, [_P, _P, _P]),
+            "ref_look_at_lh": (None, [_P, _P, _P, _P]),
+            "ref_perspective_lh_zo": (None, [_f, _f, _f, _f, _P]),
+            "ref_propagate_tree": (None, [_P, _P, _u64, _P]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _ref = L
+    return _ref
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+# ----------------------------------------------------------- wrappers
+
+def process_drawables(regions, push, level: int, n: int) -> int:
+    """regions: list of (device base, bytearray-like numpy uint8 array); outputs are
+    written into the numpy arrays in place.  push: (table, list, indirect, pointers)."""
+    arr = (Region * len(regions))()
+    for i, (base, buf) in enumerate(regions):
+        arr[i] = Region(base, buf.nbytes, buf.ctypes.data)
+    pc = (ctypes.c_uint64 * 4)(*push)
+    return lib().oracle_process_drawables(arr, len(regions), pc, level, n)
+
+
+def mat4_mul(A, B):
+    R = np.zeros(16, dtype=np.float32)
+    lib().oracle_mat4_mul(_p(np.ascontiguousarray(A, np.float32)), _p(np.ascontiguousarray(B, np.float32)), _p(R))
+    return R
+
+
+def sphere_transform(M, s):
+    o = np.zeros(4, dtype=np.float32)
+    lib().oracle_sphere_transform(_p(np.ascontiguousarray(M, np.float32)), _p(np.ascontiguousarray(s, np.float32)),
+                                  _p(o))
+    return o
+
+
+def propagate_level(local, parent, parent_world, mesh_index=None, meshes=None):
+    local = np.ascontiguousarray(local, np.float32)
+    n = local.shape[0]
+    world = np.zeros((n, 16), dtype=np.float32)
+    spheres = np.zeros((n, 4), dtype=np.float32) if mesh_index is not None else None
+    pc = 0 if parent_world is None else parent_world.shape[0]
+    lib().oracle_propagate_level(_p(local), _p(None if parent is None else np.ascontiguousarray(parent, np.uint32)),
+                                 pc, _p(None if parent_world is None else np.ascontiguousarray(parent_world)),
+                                 _p(world), n,
+                                 _p(None if mesh_index is None else np.ascontiguousarray(mesh_index, np.uint32)),
+                                 _p(meshes), 0 if meshes is None else meshes.shape[0], _p(spheres))
+    return world, spheres
+
+
+def propagate_scene(scene):
+    """All levels of a cadr_amd.scenes.Scene; returns (world of each level, leaf spheres)."""
+    worlds = []
+    prev = None
+    spheres = None
+    for d, (local, parent) in enumerate(scene.levels):
+        last = d == len(scene.levels) - 1
+        w, s = propagate_level(local, parent, prev, scene.mesh_index if last else None,
+                               scene.meshes if last else None)
+        worlds.append(w)
+        prev = w
+        if last:
+            spheres = s
+    return worlds, spheres
+
+
+def make_view(cam, view_dtype):
+    out = np.zeros(1, dtype=view_dtype)
+    eye = np.asarray(cam.eye, np.float32); d = np.asarray(cam.dir, np.float32); up = np.asarray(cam.up, np.float32)
+    th = np.asarray(cam.thresholds, np.float32)
+    lib().oracle_make_view(_p(eye), _p(d), _p(up), float(np.float32(cam.fovy)), float(np.float32(cam.aspect)),
+                           float(np.float32(cam.near)), float(np.float32(cam.far)),
+                           float(np.float32(cam.viewport_height)), _p(th), _p(out))
+    return out
+
+
+def cull(spheres, mesh_index, meshes, stateset_counts, views, stride=None, cmd_dtype=None, nthreads=1):
+    spheres = np.ascontiguousarray(spheres, np.float32)
+    n = spheres.shape[0]
+    stride = n if stride is None else stride
+    counts_in = np.ascontiguousarray(stateset_counts, np.uint32)
+    V = views.shape[0]
+    K = counts_in.shape[0]
+    cmds = np.zeros((V, stride), dtype=cmd_dtype)
+    counts = np.zeros((V, K), dtype=np.uint32)
+    lib().oracle_cull(_p(spheres), _p(np.ascontiguousarray(mesh_index, np.uint32)), _p(meshes), meshes.shape[0],
+                      _p(counts_in), K, _p(views), V, stride, _p(cmds), _p(counts), nthreads)
+    return cmds, counts

@@ -1,0 +1,73 @@
+// C entry points around the reference's own FP32 math, compiled from the
+// reference tree where it lies (src/CadR/BoundingSphere.h, 3rdParty/GLM) by
+// oracle/Makefile into oracle/_ref/libcadr_ref.so.  TEST INFRASTRUCTURE ONLY:
+// it pins the oracle's restatement of these functions bit for bit.
+#include <CadR/BoundingSphere.h>
+#include <glm/ext/matrix_clip_space.hpp>
+#include <glm/ext/matrix_transform.hpp>
+#include <glm/mat4x4.hpp>
+
+#include <cstdint>
+#include <cstring>
+
+namespace {
+
+glm::mat4 loadMat(const float* p)
+{
+	glm::mat4 m;
+	std::memcpy(&m[0][0], p, 64);
+	return m;
+}
+
+void storeMat(const glm::mat4& m, float* p) { std::memcpy(p, &m[0][0], 64); }
+
+}  // namespace
+
+extern "C" {
+
+// glm operator*(mat4, mat4), type_mat4x4.inl:630-648
+void ref_mat4_mul(const float* A, const float* B, float* R) { storeMat(loadMat(A) * loadMat(B), R); }
+
+// glm operator*(mat4, vec4), type_mat4x4.inl:536-572
+void ref_mat4_vec4(const float* M, const float* v, float* out)
+{
+	const glm::vec4 r = loadMat(M) * glm::vec4(v[0], v[1], v[2], v[3]);
+	out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+
+// CadR::operator*(mat4, BoundingSphere), BoundingSphere.h:70-87
+void ref_sphere_transform(const float* M, const float* s, float* out)
+{
+	const CadR::BoundingSphere b{glm::vec3(s[0], s[1], s[2]), s[3]};
+	const CadR::BoundingSphere r = loadMat(M) * b;
+	out[0] = r.center.x; out[1] = r.center.y; out[2] = r.center.z; out[3] = r.radius;
+}
+
+// glm::lookAtLH, ext/matrix_transform.inl:122-142
+void ref_look_at_lh(const float* eye, const float* center, const float* up, float* out)
+{
+	storeMat(glm::lookAtLH(glm::vec3(eye[0], eye[1], eye[2]), glm::vec3(center[0], center[1], center[2]),
+	                       glm::vec3(up[0], up[1], up[2])),
+	         out);
+}
+
+// glm::perspectiveLH_ZO, ext/matrix_clip_space.inl:255-268
+void ref_perspective_lh_zo(float fovy, float aspect, float zNear, float zFar, float* out)
+{
+	storeMat(glm::perspectiveLH_ZO(fovy, aspect, zNear, zFar), out);
+}
+
+// Depth-first node propagation in the manner of examples/gltfReader/main.cpp:1538-1576
+// (m = parentMatrix * node.matrix, glm operator*) for a tree given as
+// parent indices in topological order (parent[i] < i, or -1 for a root):
+// each node's world matrix is its parent's world matrix times its local one.
+void ref_propagate_tree(const float* local, const int64_t* parent, uint64_t count, float* world)
+{
+	for (uint64_t i = 0; i < count; i++) {
+		const glm::mat4 m = parent[i] < 0 ? loadMat(local + 16 * i)
+		                                  : loadMat(world + 16 * parent[i]) * loadMat(local + 16 * i);
+		storeMat(m, world + 16 * i);
+	}
+}
+
+}  // extern "C"

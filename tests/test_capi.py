@@ -1,0 +1,99 @@
+"""The C ABI library loads, exports every entry point include/*.h declares, and
+its host-only functions (views, tiles, argument checks) behave without a GPU."""
+import ctypes
+import glob
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from cadr_amd import _native as N
+from cadr_amd import cull, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(cadr[A-Z]\w*)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.load()
+    names = declared_functions()
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [n for n in names if n not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+    for n in names:
+        getattr(lib, n)
+    # every binding in _native matches a declaration
+    assert set(N.SIGNATURES) <= set(names)
+
+
+def test_make_view_matches_oracle_bit_exact():
+    cams = scenes.batch_cameras(256) + [scenes.DEFAULT_CAMERA]
+    for c in cams:
+        assert cull.make_view(c).tobytes() == oracle.make_view(c, N.VIEW_DTYPE).tobytes()
+
+
+def test_make_view_rejects_bad_cameras():
+    bad = [scenes.Camera(eye=(0.0, 0.0, 0.0), dir=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fovy=1.0),  # up // dir
+           scenes.Camera(eye=(0.0, 0.0, 0.0), dir=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fovy=0.0),
+           scenes.Camera(eye=(0.0, 0.0, 0.0), dir=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fovy=1.0),
+           scenes.Camera(eye=(0.0, 0.0, 0.0), dir=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fovy=1.0, near=5.0, far=1.0)]
+    for c in bad:
+        with pytest.raises(N.CadrError) as e:
+            cull.make_view(c)
+        assert e.value.code == N.ERROR_INVALID_VALUE
+
+
+def test_build_tiles_layout():
+    counts = np.array([0, 1, 1024, 1025, 3000, 0], np.uint32)
+    t = cull.build_tiles(counts, 1024)
+    base = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    # every drawable covered once, tiles within their StateSet, flags at the ends
+    covered = np.zeros(int(counts.sum()), np.int64)
+    per_k = {}
+    for tile in t:
+        k = int(tile["stateSetAndFlags"]) & N.TILE_STATESET_MASK
+        per_k.setdefault(k, []).append(tile)
+        assert tile["stateSetBase"] == base[k]
+        assert base[k] <= tile["start"] and tile["start"] + tile["count"] <= base[k + 1]
+        assert tile["count"] <= 1024
+        covered[tile["start"]: tile["start"] + tile["count"]] += 1
+    assert np.all(covered == 1)
+    for k in range(len(counts)):
+        ts = per_k[k]
+        assert int(ts[0]["stateSetAndFlags"]) & N.TILE_FIRST and int(ts[-1]["stateSetAndFlags"]) & N.TILE_LAST
+        assert len(ts) == max(1, -(-int(counts[k]) // 1024))
+        if counts[k] == 0:
+            assert ts[0]["count"] == 0
+    # counting mode agrees; a short buffer is an error
+    n = ctypes.c_uint32(0)
+    assert N.load().cadrBuildCullTiles(N.ptr(counts), counts.size, 1024, None, 0, ctypes.byref(n)) == 0
+    assert n.value == len(t)
+    small = np.zeros(2, N.TILE_DTYPE)
+    assert N.load().cadrBuildCullTiles(N.ptr(counts), counts.size, 1024, N.ptr(small), 2, ctypes.byref(n)) \
+        == N.ERROR_INVALID_VALUE
+
+
+def test_cull_config_tile_sizes():
+    for (b, i) in [(256, 2), (256, 4), (256, 8), (512, 2), (512, 4), (128, 8), (128, 16)]:
+        assert cull.tile_size(cull.make_config(b, i)) == b * i
+    assert cull.tile_size(cull.make_config(100, 3)) == 0
+
+
+def test_result_strings():
+    lib = N.load()
+    assert lib.cadrGetResultString(0) == b"CADR_SUCCESS"
+    assert lib.cadrGetResultString(2) == b"CADR_ERROR_OUT_OF_RESOURCES"

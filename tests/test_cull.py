@@ -1,0 +1,192 @@
+"""Transform propagation, world spheres and the cull / LOD / compaction pass:
+GPU results against the CPU oracle, bit for bit."""
+import numpy as np
+import pytest
+
+import oracle
+from cadr_amd import _native as N
+from cadr_amd import cull, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(cmds, counts, exp_cmds, exp_counts, stateset_counts, what=""):
+    assert np.array_equal(counts, exp_counts), f"{what}: counts differ, first at {np.argwhere(counts != exp_counts)[:5]}"
+    got = cull.valid_commands(cmds, counts, stateset_counts)
+    exp = cull.valid_commands(exp_cmds, exp_counts, stateset_counts)
+    for v in range(len(got)):
+        for k in range(len(stateset_counts)):
+            if got[v][k].tobytes() != exp[v][k].tobytes():
+                a, b = got[v][k], exp[v][k]
+                bad = np.argwhere(a != b)[:5]
+                raise AssertionError(f"{what}: view {v} StateSet {k} commands differ at {bad}: {a[bad]} vs {b[bad]}")
+
+
+def _run(ds, views, config, stride=None):
+    out = cull.CullOutputs(views.shape[0], stride or ds.n, ds.K)
+    b = cull.CullBatch(config, [(ds, views, out)])
+    b.run()
+    from cadr_amd.device import synchronize
+    synchronize()
+    cmds, counts = out.download()
+    b.close(); out.close()
+    return cmds, counts
+
+
+def test_propagation_and_spheres_bit_exact(gpu):
+    scene = scenes.tree_scene((10, 100, 100), seed=scenes.SEED_BASE + 11)
+    print("propagation 3-level 100k leaves", flush=True)
+    ds = cull.DeviceScene(scene)
+    ds.propagate()
+    worlds, spheres = oracle.propagate_scene(scene)
+    for d in range(3):
+        assert ds.world_matrices(d).tobytes() == worlds[d].tobytes(), f"level {d} world matrices differ"
+    got = ds.download_spheres()
+    bad = np.argwhere(got.view(np.uint32) != spheres.view(np.uint32))
+    assert bad.size == 0, f"spheres differ at {bad[:5]}: {got[bad[:5, 0]]} vs {spheres[bad[:5, 0]]}"
+    ds.close()
+
+
+def test_world_spheres_flat_bit_exact(gpu):
+    from cadr_amd.device import DeviceBuffer, synchronize
+    import ctypes
+    scene = scenes.plain_scene(100_000)
+    world = scene.levels[0][0]
+    # include unusual matrices: shear, mirror, huge and tiny scales, an invalid mesh index
+    rng = np.random.default_rng(5)
+    world = world.copy()
+    world[:1000] = (rng.standard_normal((1000, 16)) * 10).astype(np.float32)
+    mi = scene.mesh_index.copy()
+    mi[7] = 10_000
+    wb, mb, meb = DeviceBuffer.from_array(world), DeviceBuffer.from_array(mi), DeviceBuffer.from_array(scene.meshes)
+    sb = DeviceBuffer(16 * world.shape[0])
+    N.check(N.load().cadrComputeWorldSpheres(wb.address, mb.address, meb.address, scene.meshes.shape[0],
+                                             world.shape[0], sb.address, None))
+    synchronize()
+    got = sb.download(np.float32, 4 * world.shape[0]).reshape(-1, 4)
+    _, exp = oracle.propagate_level(world, None, None, mi, scene.meshes)
+    assert got.tobytes() == exp.tobytes()
+    assert got[7, 3] == -np.inf
+
+
+@pytest.fixture(scope="module")
+def plain(gpu):
+    scene = scenes.plain_scene(100_000)
+    ds = cull.DeviceScene(scene)
+    ds.propagate()
+    _, spheres = oracle.propagate_scene(scene)
+    assert ds.download_spheres().tobytes() == spheres.tobytes()
+    yield scene, ds, spheres
+    ds.close()
+
+
+@pytest.mark.parametrize("block,items", [(256, 2), (256, 4), (256, 8), (512, 2), (512, 4), (128, 8), (128, 16)])
+@pytest.mark.parametrize("flags", [0, N.CULL_NO_TILE_SPHERES])
+def test_cull_plain100k_single_view(plain, block, items, flags):
+    scene, ds, spheres = plain
+    print(f"plain100k block={block} items={items} flags={flags}", flush=True)
+    views = cull.make_views([scenes.DEFAULT_CAMERA])
+    cmds, counts = _run(ds, views, cull.make_config(block, items, flags=flags))
+    exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
+                                       cmd_dtype=N.CMD_DTYPE)
+    assert exp_counts.sum() > 1000
+    _compare(cmds, counts, exp_cmds, exp_counts, scene.stateset_counts, "plain100k")
+
+
+def test_cull_views_match_oracle_views(plain):
+    cams = scenes.batch_cameras(64)
+    for c in cams:
+        a = cull.make_view(c)
+        b = oracle.make_view(c, N.VIEW_DTYPE)
+        assert a.tobytes() == b.tobytes()
+
+
+def test_cull_many_views_and_statesets(gpu):
+    """300 views (crosses the 64-view look-back groups) on a 2-level 200k scene with 64 StateSets."""
+    scene = scenes.tree_scene((2000, 100), seed=scenes.SEED_BASE + 12, num_statesets=64, lod_count=3,
+                              trange=200.0)
+    ds = cull.DeviceScene(scene)
+    ds.propagate()
+    _, spheres = oracle.propagate_scene(scene)
+    assert ds.download_spheres().tobytes() == spheres.tobytes()
+    cams = scenes.random_cameras(scenes.SEED_BASE + 12, 20, 0, 300, box=400.0)
+    views = cull.make_views(cams)
+    exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
+                                       cmd_dtype=N.CMD_DTYPE, nthreads=8)
+    assert exp_counts.sum() > 10000
+    for block, items, flags in [(256, 4, 0), (256, 8, 0), (512, 2, N.CULL_NO_TILE_SPHERES)]:
+        print(f"300 views block={block} items={items} flags={flags}", flush=True)
+        cmds, counts = _run(ds, views, cull.make_config(block, items, flags=flags))
+        _compare(cmds, counts, exp_cmds, exp_counts, scene.stateset_counts, f"{block}x{items}")
+    ds.close()
+
+
+def test_cull_uneven_batch(gpu):
+    """Several scenes of different sizes, depths and LOD counts in one launch."""
+    specs = [(3000, 1, 1, 3), (50_000, 2, 4, 17), (120_000, 3, 2, 1), (1, 1, 1, 5), (70_000, 4, 3, 65)]
+    entries, exp = [], []
+    for i, (n, depth, lods, nv) in enumerate(specs):
+        sc = scenes.uneven_scene(i, n, depth, lods, nv)
+        ds = cull.DeviceScene(sc)
+        ds.propagate()
+        _, sph = oracle.propagate_scene(sc)
+        assert ds.download_spheres().tobytes() == sph.tobytes(), f"scene {i} spheres"
+        views = cull.make_views(sc.cameras)
+        out = cull.CullOutputs(nv, ds.n, ds.K)
+        entries.append((ds, views, out))
+        exp.append(oracle.cull(sph, sc.mesh_index, sc.meshes, sc.stateset_counts, views, cmd_dtype=N.CMD_DTYPE,
+                               nthreads=8) + (sc,))
+    from cadr_amd.device import synchronize
+    for rep in range(2):  # a second launch reuses status words under a new epoch
+        b = cull.CullBatch(cull.make_config(256, 4), entries)
+        b.run()
+        synchronize()
+        for (ds, views, out), (ec, en, sc) in zip(entries, exp):
+            cmds, counts = out.download()
+            _compare(cmds, counts, ec, en, sc.stateset_counts, f"{sc.name} rep {rep}")
+        b.close()
+    for ds, _, out in entries:
+        out.close(); ds.close()
+
+
+def test_cull_edge_cases(gpu):
+    from cadr_amd.device import synchronize
+    rng = np.random.default_rng(3)
+    meshes = scenes.make_meshes(77, 4, [1, 2, 3, 4])
+    cam = scenes.Camera(eye=(0.0, 0.0, 0.0), dir=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fovy=1.2)
+    views = cull.make_views([cam, scenes.Camera(eye=(0.0, 0.0, 0.0), dir=(0.0, 0.0, 1.0), up=(0.0, 1.0, 0.0),
+                                                fovy=1.2)])
+    cases = {
+        "empty statesets": np.array([0, 5, 0, 0, 3000, 0], np.uint32),
+        "all empty": np.array([0, 0, 0], np.uint32),
+        "single": np.array([1], np.uint32),
+        "tile boundaries": np.array([1023, 1024, 1025, 2047, 2048, 2049, 4097], np.uint32),
+        "many small": rng.integers(0, 9, 700).astype(np.uint32),
+    }
+    for name, counts_in in cases.items():
+        n = int(counts_in.sum())
+        c = rng.uniform(-60, 60, (n, 3)).astype(np.float32)
+        c[: n // 3, 2] = -np.abs(c[: n // 3, 2]) - 5  # a third in front of the first camera
+        r = rng.uniform(0.1, 3, (n, 1)).astype(np.float32)
+        s = np.concatenate([c, r], axis=1).astype(np.float32)
+        mi = rng.integers(0, 5, n).astype(np.uint32)  # 4 = invalid mesh index
+        if n:
+            s[0] = (0.0, 0.0, 0.0, 1.0)  # contains the eye
+        sc = scenes.Scene(name, [(np.zeros((max(n, 1), 16), np.float32), None)], mi, meshes, counts_in, [cam])
+        ds = cull.DeviceScene(sc, keep_levels=False)
+        if n:
+            ds.upload_spheres(s)
+        exp_cmds, exp_counts = oracle.cull(s, mi, meshes, counts_in, views, cmd_dtype=N.CMD_DTYPE)
+        for block, items in [(256, 4), (128, 8)]:
+            print(f"edge {name} {block}x{items}", flush=True)
+            out = cull.CullOutputs(2, max(n, 1), len(counts_in))
+            out.counts.fill_u32(0xDEADBEEF)
+            synchronize()
+            b = cull.CullBatch(cull.make_config(block, items), [(ds, views, out)])
+            b.run()
+            synchronize()
+            cmds, counts = out.download()
+            _compare(cmds[:, :max(n, 1)], counts, exp_cmds if n else np.zeros((2, 1), N.CMD_DTYPE), exp_counts,
+                     counts_in, name)
+            b.close(); out.close()
+        ds.close()

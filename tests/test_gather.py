@@ -1,0 +1,87 @@
+"""Gather pass (processDrawables.comp:92-113): oracle pinning on the CPU and
+kernel parity on the GPU, handle levels 1-3, table boundaries and dispatch
+sizes around the reference's 32768-workgroup split (Renderer.cpp:684-692)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from cadr_amd import _native as N
+from gather_fixtures import build_gather_scene
+
+
+def _oracle_run(img, push, n, level):
+    rc = oracle.process_drawables([(img.base, img.buf)], push, level, n)
+    ind = img.read(push[2], np.uint32, 4 * n).reshape(n, 4)
+    ptr = img.read(push[3], np.uint64, 4 * n).reshape(n, 4)
+    return rc, ind, ptr
+
+
+@pytest.mark.parametrize("level,n", [(1, 1), (1, 500), (2, 0), (2, 1), (2, 3000), (3, 2000)])
+def test_oracle_gather_matches_fixture_truth(level, n):
+    img, push, exp_ind, exp_ptr = build_gather_scene(1 << 40, level, n, np.random.default_rng(level * 100 + n))
+    rc, ind, ptr = _oracle_run(img, push, n, level)
+    assert rc == 0
+    assert np.array_equal(ind, exp_ind)
+    assert np.array_equal(ptr, exp_ptr)
+
+
+def test_oracle_lookup_levels_closed_form():
+    """Handle h at level 2 is table[h>>11][h&0x7ff]; level 3 table[h>>22][(h>>11)&0x7ff][h&0x7ff]."""
+    from gather_fixtures import HandleTables, Image
+    for level, h in [(1, 2047), (2, 2048), (2, 4194303), (3, 4194304), (3, (7 << 22) + (2047 << 11) + 1)]:
+        img = Image(0x10000, 1 << 20)
+        t = HandleTables(img, level)
+        t.set(h, 0xABCDEF00)
+        out = ctypes.c_uint64(0)
+        reg = (oracle.Region * 1)(oracle.Region(img.base, img.buf.nbytes, img.buf.ctypes.data))
+        assert oracle.lib().oracle_lookup_handle(reg, 1, t.root, level, h, ctypes.byref(out)) == 0
+        assert out.value == 0xABCDEF00
+        # a neighbouring handle in an absent table resolves to 0
+        assert oracle.lib().oracle_lookup_handle(reg, 1, t.root, level, h ^ (1 << 11) if level > 1 else 3,
+                                                 ctypes.byref(out)) == 0
+        assert out.value == 0
+
+
+def _gpu_gather(level, n, seed, **kw):
+    from cadr_amd.device import DeviceBuffer, synchronize
+    probe, *_ = build_gather_scene(0, level, n, np.random.default_rng(seed), **kw)
+    buf = DeviceBuffer(probe.buf.size)
+    img, push, exp_ind, exp_ptr = build_gather_scene(buf.address, level, n, np.random.default_rng(seed),
+                                                     image_size=probe.buf.size, **kw)
+    buf.upload(img.buf)
+    pc = N.PushConstants(*push)
+    N.check(N.load().cadrProcessDrawables(ctypes.byref(pc), level, n, None))
+    synchronize()
+    got = buf.download(np.uint8)
+    base = buf.address
+    ind = got[push[2] - base: push[2] - base + 16 * n].view(np.uint32).reshape(n, 4)
+    ptr = got[push[3] - base: push[3] - base + 32 * n].view(np.uint64).reshape(n, 4)
+    rc, o_ind, o_ptr = _oracle_run(img, push, n, level)
+    buf.close()
+    return rc, ind, ptr, o_ind, o_ptr, exp_ind, exp_ptr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level,n", [(1, 1), (1, 600), (2, 255), (2, 256), (2, 257), (2, 32768), (2, 32769),
+                                     (3, 65537), (3, 100_000)])
+def test_gpu_gather_parity(gpu, level, n):
+    print(f"gather level={level} n={n}", flush=True)
+    rc, ind, ptr, o_ind, o_ptr, exp_ind, exp_ptr = _gpu_gather(level, n, 1000 + level * 7 + n,
+                                                               with_drawable_data=(level != 1 or n < 600))
+    assert rc == 0
+    assert np.array_equal(o_ind, exp_ind) and np.array_equal(o_ptr, exp_ptr)
+    assert np.array_equal(ind, o_ind), f"indirect mismatch at {np.argwhere(ind != o_ind)[:5]}"
+    assert np.array_equal(ptr, o_ptr), f"pointers mismatch at {np.argwhere(ptr != o_ptr)[:5]}"
+
+
+@pytest.mark.gpu
+def test_gpu_gather_zero_and_errors(gpu):
+    pc = N.PushConstants(0, 0, 0, 0)
+    assert N.load().cadrProcessDrawables(ctypes.byref(pc), 2, 0, None) == N.SUCCESS  # no-op
+    assert N.load().cadrProcessDrawables(ctypes.byref(pc), 2, 5, None) == N.ERROR_INVALID_VALUE
+    pc = N.PushConstants(64, 64, 64, 64)
+    assert N.load().cadrProcessDrawables(ctypes.byref(pc), 4, 5, None) == N.ERROR_INVALID_VALUE
+    pc = N.PushConstants(64, 72, 64, 64)  # misaligned drawable list
+    assert N.load().cadrProcessDrawables(ctypes.byref(pc), 2, 5, None) == N.ERROR_INVALID_VALUE
