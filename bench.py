@@ -1,0 +1,338 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X drawable-processing pass.
+
+Default run (one GPU): the full10m workload — a 3-level transform tree of
+10,101,000 nodes (1,000 roots -> 100 children -> 100 leaves), 10M leaf
+drawables in 64 StateSets, 3 LODs — propagated level by level to world
+matrices and world spheres, then frustum-culled, LOD-selected and compacted
+per StateSet for one view.  A step is one such frame.  The line also reports
+the batch256 workload (the same scene culled for 256 views in one persistent
+launch, replayed from a HIP graph) against the plain pass run once per view.
+
+Prints one JSON line (metric / value / roofline / cpu_baseline ...).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--block", type=int, default=256)
+    p.add_argument("--items", type=int, default=4)
+    p.add_argument("--batch-views", type=int, default=256)
+    p.add_argument("--batch-steps", type=int, default=10)
+    p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--scale", type=float, default=1.0, help="root-count scale of full10m (1.0 = 10M drawables)")
+    return p.parse_args()
+
+
+class Timer:
+    """HIP events around one kernel launch on the stream, summed over steps."""
+
+    def __init__(self, n):
+        from cadr_amd.device import Event
+        self.starts = [Event() for _ in range(n)]
+        self.ends = [Event() for _ in range(n)]
+        self.i = 0
+
+    def begin(self, stream):
+        self.starts[self.i].record(stream)
+
+    def end(self, stream):
+        self.ends[self.i].record(stream)
+        self.i += 1
+
+    def mean_ms(self):
+        return sum(s.elapsed_ms(e) for s, e in zip(self.starts[: self.i], self.ends[: self.i])) / max(self.i, 1)
+
+
+def main():
+    args = parse()
+    # keep large numpy temporaries in the heap instead of fresh mmaps (first-touch page faults)
+    try:
+        libc = ctypes.CDLL("libc.so.6")
+        libc.mallopt(-1, 1 << 40)   # M_TRIM_THRESHOLD
+        libc.mallopt(-3, 32 << 20)  # M_MMAP_THRESHOLD
+    except OSError:
+        pass
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; the pass shards by scene range or by view "
+                                   "(see DESIGN.md), and no multi-GPU figure is produced", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from cadr_amd import _native as N
+    from cadr_amd import cull, device, scenes
+
+    device.set_device(0)
+    stream = device.Stream()
+    s = stream.handle
+
+    t0 = time.time()
+    spec = scenes.full10m_spec(args.scale)
+    ds = cull.DeviceScene(spec)  # streamed to HBM level by level
+    log(f"scene generated and uploaded in {time.time() - t0:.1f}s: {ds.n} drawables, {sum(spec.sizes)} nodes")
+    n = ds.n
+    config = cull.make_config(args.block, args.items)
+    views1 = cull.make_views([scenes.DEFAULT_CAMERA])
+    out1 = cull.CullOutputs(1, n, ds.K)
+    batch1 = cull.CullBatch(config, [(ds, views1, out1)])
+
+    lib = N.load()
+    level_descs = []
+    prev = None
+    for d, (cnt, lb, pb, wb) in enumerate(ds.levels):
+        lv = np.zeros(1, dtype=N.LEVEL_DTYPE)
+        lv["localMatrices"], lv["worldMatrices"], lv["count"] = lb.address, wb.address, cnt
+        if prev is not None:
+            lv["parentIndex"], lv["parentWorld"], lv["parentCount"] = pb.address, prev[1].address, prev[0]
+        if d == len(ds.levels) - 1:
+            lv["meshIndex"], lv["meshes"], lv["numMeshes"], lv["spheres"] = (ds.mesh_index.address, ds.meshes.address,
+                                                                           ds.num_meshes, ds.spheres.address)
+        level_descs.append(lv)
+        prev = (cnt, wb)
+
+    total = args.warmup + args.steps
+    t_leaf = Timer(total)
+    t_cull = Timer(total)
+
+    def frame():
+        for d, lv in enumerate(level_descs):
+            last = d == len(level_descs) - 1
+            if last:
+                t_leaf.begin(s)
+            N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
+            if last:
+                t_leaf.end(s)
+        t_cull.begin(s)
+        batch1.run(s)
+        t_cull.end(s)
+
+    for _ in range(args.warmup):
+        frame()
+    stream.synchronize()
+    # the timed region: exactly `steps` frames, synchronised on both sides
+    t_leaf.starts, t_leaf.ends = t_leaf.starts[args.warmup:], t_leaf.ends[args.warmup:]
+    t_cull.starts, t_cull.ends = t_cull.starts[args.warmup:], t_cull.ends[args.warmup:]
+    t_leaf.i = t_cull.i = 0
+    device.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame()
+    stream.synchronize()
+    device.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    _, counts1 = out1.download()
+    survivors1 = int(counts1.sum())
+    leaf_ms = t_leaf.mean_ms()
+    cull_ms = t_cull.mean_ms()
+
+    # algorithmic bytes of the leaf propagation launch (the dominant kernel):
+    # per leaf 64 B local + 4 B parent index + 4 B mesh index read, 64 B world
+    # + 16 B sphere written; plus each parent's world matrix read once.
+    leaves = ds.levels[-1][0]
+    parents = ds.levels[-2][0] if len(ds.levels) > 1 else 0
+    leaf_bytes = leaves * (64 + 4 + 4 + 64 + 16) + parents * 64
+    leaf_gbs = leaf_bytes / (leaf_ms * 1e-3) / 1e9
+    inner_nodes = sum(l[0] for l in ds.levels[:-1])
+    frame_bytes = leaf_bytes + inner_nodes * (64 + 4 + 64) + n * 20 + survivors1 * 20 + ds.K * 4
+    cull_bytes = n * 20 + survivors1 * 20 + ds.K * 4
+    result = {
+        "metric": "culled drawables/s + % of HBM roofline; p50/p99 view latency (served)",
+        "value": n / (ms_per_step * 1e-3),
+        "unit": "drawables/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 scene generator)",
+        "config": {"workload": "full10m: 1000x100x100 transform tree, 10M drawables, 64 StateSets, 3 LODs, "
+                               "1 view (propagation + cull + LOD + per-StateSet compaction)",
+                   "drawables": n, "nodes": int(sum(l[0] for l in ds.levels)), "statesets": ds.K, "views": 1,
+                   "block": args.block, "items_per_thread": args.items},
+        "roofline": {"bound": "hbm", "kernel": "propagateKernel<false,true> (leaf level + world spheres)",
+                     "achieved": leaf_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": leaf_gbs / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": leaf_bytes, "avg_ms": leaf_ms},
+        "frame": {"algorithmic_bytes": frame_bytes, "hbm_frac": frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                  "cull_ms": cull_ms, "cull_gbs": cull_bytes / (cull_ms * 1e-3) / 1e9, "visible": survivors1},
+    }
+    log(f"full10m: {ms_per_step:.3f} ms/frame, leaf {leaf_ms:.3f} ms ({leaf_gbs:.0f} GB/s), "
+        f"cull {cull_ms:.3f} ms, visible {survivors1}")
+
+    if not args.no_batch:
+        result["batched"] = bench_batched(args, ds, config, stream, lib, N, cull, scenes, device)
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+
+
+def bench_batched(args, ds, config, stream, lib, N, cull, scenes, device):
+    """batch256: all views in one persistent launch (graph-replayed) vs the plain
+    pass launched once per view."""
+    s = stream.handle
+    V = args.batch_views
+    cams = scenes.batch_cameras(V)
+    views = cull.make_views(cams)
+    n = ds.n
+    out = cull.CullOutputs(V, n, ds.K)
+    batch = cull.CullBatch(config, [(ds, views, out)])
+    # plain: one single-view launch per view
+    ws = cull.Workspace(max(batch.status_words, 1), 4)
+    # per-view jobs that write straight into the batched output arrays (same layout)
+    plain_jobs = []
+    for v in range(V):
+        j = batch.host_jobs.copy()
+        j["views"] = batch.host_jobs["views"][0] + v * N.VIEW_DTYPE.itemsize
+        j["commands"] = out.commands.address + v * n * 20
+        j["counts"] = out.counts.address + v * ds.K * 4
+        j["numViews"] = 1
+        plain_jobs.append((j, device.DeviceBuffer.from_array(j)))
+
+    def run_plain():
+        for j, dj in plain_jobs:
+            N.check(lib.cadrCullBatch(ctypes.byref(config), dj.address, N.ptr(j), 1, ws.handle, s))
+
+    for _ in range(2):
+        run_plain()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    reps = max(1, args.batch_steps // 2)
+    for _ in range(reps):
+        run_plain()
+    stream.synchronize()
+    plain_ms = 1e3 * (time.perf_counter() - t0) / reps
+    _, counts_plain = out.download()
+
+    # batched, captured once into a HIP graph and replayed
+    graph = Graph()
+    graph.begin(s)
+    batch.run(s)
+    graph.end(s)
+    for _ in range(2):
+        graph.launch(s)
+    stream.synchronize()
+    from cadr_amd.device import Event
+    e0, e1 = Event(), Event()
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(args.batch_steps):
+        graph.launch(s)
+    e1.record(s)
+    stream.synchronize()
+    batch_ms = 1e3 * (time.perf_counter() - t0) / args.batch_steps
+    batch_kernel_ms = e0.elapsed_ms(e1) / args.batch_steps
+    _, counts = out.download()
+    same = bool(np.array_equal(counts, counts_plain))
+    survivors = int(counts.sum())
+    tests = V * n
+    alg_bytes = n * 20 + survivors * 20 + V * ds.K * 4
+    gbs = alg_bytes / (batch_kernel_ms * 1e-3) / 1e9
+    res = {
+        "workload": f"batch256: full10m scene x {V} views (eye U[-600,600]^3, random direction, vfov U[40,90] deg), "
+                    "one persistent launch per frame, HIP-graph replay",
+        "views": V, "tests_per_frame": tests, "visible": survivors,
+        "ms_per_frame": batch_ms, "kernel_ms": batch_kernel_ms,
+        "culled_drawables_per_s": tests / (batch_ms * 1e-3),
+        "roofline": {"bound": "hbm", "kernel": f"cullKernel<{config.blockSize},{config.itemsPerThread}>",
+                     "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "algorithmic_bytes": alg_bytes},
+        "plain_ms_per_frame": plain_ms,
+        "plain_culled_drawables_per_s": tests / (plain_ms * 1e-3),
+        "speedup_vs_plain": plain_ms / batch_ms,
+        "plain_and_batched_counts_identical": same,
+    }
+    log(f"batch{V}: {batch_ms:.3f} ms/frame ({tests / (batch_ms * 1e-3) / 1e9:.1f} G tests/s, {gbs:.0f} GB/s alg), "
+        f"plain {plain_ms:.3f} ms, speedup {plain_ms / batch_ms:.2f}x, identical={same}")
+    graph.close()
+    batch.close(); out.close(); ws.close()
+    for _, dj in plain_jobs:
+        dj.close()
+    return res
+
+
+class Graph:
+    """hipStreamBeginCapture / hipGraphInstantiate / hipGraphLaunch through libamdhip64."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.graph = ctypes.c_void_p(0)
+        self.exec = ctypes.c_void_p(0)
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: {rc}")
+
+    def begin(self, stream):
+        self._ck(self.hip.hipStreamBeginCapture(ctypes.c_void_p(stream), 0), "hipStreamBeginCapture")
+
+    def end(self, stream):
+        self._ck(self.hip.hipStreamEndCapture(ctypes.c_void_p(stream), ctypes.byref(self.graph)), "hipStreamEndCapture")
+        self._ck(self.hip.hipGraphInstantiate(ctypes.byref(self.exec), self.graph, None, None, ctypes.c_size_t(0)),
+                 "hipGraphInstantiate")
+
+    def launch(self, stream):
+        self._ck(self.hip.hipGraphLaunch(self.exec, ctypes.c_void_p(stream)), "hipGraphLaunch")
+
+    def close(self):
+        if self.exec:
+            self.hip.hipGraphExecDestroy(self.exec)
+        if self.graph:
+            self.hip.hipGraphDestroy(self.graph)
+        self.exec = self.graph = None
+
+
+def cpu_baseline(scale: float = 0.25, budget_s: float = 10.0):
+    """The oracle (C restatement, gcc -O2, one thread) on a bounded sample of
+    the workload: the full10m structure with a quarter of the roots (2.5M
+    drawables), whole frames (propagation + cull of one view) repeated until
+    about budget_s seconds of CPU work; median frame time."""
+    import oracle
+    from cadr_amd import _native as N
+    from cadr_amd import cull, scenes
+    scene = scenes.full10m_scene(scale)
+    views = cull.make_views([scenes.DEFAULT_CAMERA])
+    times = []
+    start = time.perf_counter()
+    while len(times) < 3 or (time.perf_counter() - start < budget_s and len(times) < 50):
+        t0 = time.perf_counter()
+        _, spheres = oracle.propagate_scene(scene)
+        oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views, cmd_dtype=N.CMD_DTYPE,
+                    nthreads=1)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    log(f"cpu baseline: {t:.3f} s per {scene.num_drawables}-drawable frame ({len(times)} frames)")
+    return {"value": scene.num_drawables / t, "unit": "drawables/s", "cores": 1, "kind": "port",
+            "sample": f"full10m structure at {scale:g} of the roots ({scene.num_drawables} drawables), whole frames "
+                      f"(propagation + cull of 1 view), median of {len(times)}; oracle/cadr_oracle.c, gcc -O2, "
+                      "one thread"}
+
+
+if __name__ == "__main__":
+    main()

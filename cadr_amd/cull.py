@@ -55,7 +55,8 @@ def build_tiles(stateset_counts, tile: int) -> np.ndarray:
 class DeviceScene:
     """A scene resident in HBM: transform levels, world spheres, mesh table."""
 
-    def __init__(self, scene: Scene, keep_levels: bool = True):
+    def __init__(self, scene, keep_levels: bool = True, chunk: int = 1 << 18):
+        """scene: a Scene (host arrays) or a TreeSpec (streamed level by level in chunks)."""
         self.scene = scene
         self.n = scene.num_drawables
         self.K = int(len(scene.stateset_counts))
@@ -65,12 +66,24 @@ class DeviceScene:
         self.mesh_index = DeviceBuffer.from_array(scene.mesh_index)
         self.spheres = DeviceBuffer(max(16 * self.n, 16))
         self.levels = []
-        if keep_levels:
+        if keep_levels and isinstance(scene, Scene):
             for local, parent in scene.levels:
                 cnt = local.shape[0]
                 lb = DeviceBuffer.from_array(local)
                 wb = DeviceBuffer(max(64 * cnt, 16))
                 pb = DeviceBuffer.from_array(parent) if parent is not None else None
+                self.levels.append((cnt, lb, pb, wb))
+        elif keep_levels:
+            for d, cnt in enumerate(scene.sizes):
+                lb = DeviceBuffer(max(64 * cnt, 16))
+                wb = DeviceBuffer(max(64 * cnt, 16))
+                pb = DeviceBuffer(max(4 * cnt, 16)) if d > 0 else None
+                for c0 in range(0, cnt, chunk):
+                    c1 = min(cnt, c0 + chunk)
+                    local, parent = scene.level_chunk(d, c0, c1)
+                    lb.upload(local, offset=64 * c0)
+                    if pb is not None:
+                        pb.upload(parent, offset=4 * c0)
                 self.levels.append((cnt, lb, pb, wb))
         self._tiles = {}
 

@@ -46,27 +46,37 @@ def _mix(z: np.ndarray) -> np.ndarray:
     return z ^ (z >> np.uint64(31))
 
 
-def rand_u64(seed: int, stream: int, start: int, count: int, comps: int = 1) -> np.ndarray:
-    """SplitMix64 outputs for items start..start+count-1, comps values each
-    (shape (count, comps)).  Counter = stream<<40 | item*comps + comp."""
+def rand_u64(seed: int, stream: int, start: int, count: int, comps: int = 1, items=None) -> np.ndarray:
+    """SplitMix64 outputs for items start..start+count-1 (or the given item
+    indices), comps values each, shape (count, comps).
+    Counter = (stream << 40) + item*comps + comp + 1."""
     with np.errstate(over="ignore"):
-        idx = np.arange(start * comps, (start + count) * comps, dtype=np.uint64)
+        if items is None:
+            idx = np.arange(start * comps, (start + count) * comps, dtype=np.uint64)
+        else:
+            items = np.asarray(items, dtype=np.uint64)
+            count = items.shape[0]
+            idx = (items[:, None] * np.uint64(comps) + np.arange(comps, dtype=np.uint64)[None, :]).reshape(-1)
         ctr = (np.uint64(stream) << np.uint64(40)) + idx + np.uint64(1)
         x = np.uint64(seed) + ctr * GAMMA
         return _mix(x).reshape(count, comps)
 
 
-def rand_unit(seed: int, stream: int, start: int, count: int, comps: int = 1) -> np.ndarray:
+def rand_unit(seed: int, stream: int, start: int, count: int, comps: int = 1, items=None) -> np.ndarray:
     """Uniform floats in [0, 1) from the top 24 bits (float32)."""
-    u = rand_u64(seed, stream, start, count, comps) >> np.uint64(40)
+    u = rand_u64(seed, stream, start, count, comps, items) >> np.uint64(40)
     return (u.astype(np.float32) * np.float32(1.0 / 16777216.0)).astype(np.float32)
 
 
 def rand_int(seed: int, stream: int, start: int, count: int, lo: int, hi: int) -> np.ndarray:
     """Uniform integers in [lo, hi] (inclusive) from the top 32 bits."""
-    u = rand_u64(seed, stream, start, count, 1)[:, 0] >> np.uint64(32)
+    out = np.empty(count, dtype=np.int64)
     span = np.uint64(hi - lo + 1)
-    return (lo + ((u * span) >> np.uint64(32)).astype(np.int64)).astype(np.int64)
+    for c0 in range(0, count, 1 << 22):
+        c1 = min(count, c0 + (1 << 22))
+        u = rand_u64(seed, stream, start + c0, c1 - c0, 1)[:, 0] >> np.uint64(32)
+        out[c0:c1] = lo + ((u * span) >> np.uint64(32)).astype(np.int64)
+    return out
 
 
 # ------------------------------------------------------------------ meshes
@@ -94,10 +104,26 @@ def make_meshes(seed: int, count: int, lod_counts) -> np.ndarray:
 
 # --------------------------------------------------------------- matrices
 
-def rotation_translation(seed: int, stream: int, start: int, count: int, trange: float) -> np.ndarray:
+CHUNK = 1 << 18   # keeps numpy temporaries under glibc's mmap threshold
+
+
+def rotation_translation(seed: int, stream: int, start: int, count: int, trange: float, items=None) -> np.ndarray:
     """count x 16 float32 column-major matrices: uniform random rotation
-    (Shoemake's quaternion from three uniforms) and translation U[-t, t]^3."""
-    u = rand_unit(seed, stream, start, count, 6).astype(np.float64)
+    (Shoemake's quaternion from three uniforms) and translation U[-t, t]^3.
+    Matrix i is a pure function of its item index (start + i, or items[i])."""
+    if items is not None:
+        count = len(items)
+    out = np.empty((count, 16), dtype=np.float32)
+    for c0 in range(0, count, CHUNK):
+        c1 = min(count, c0 + CHUNK)
+        it = None if items is None else items[c0:c1]
+        out[c0:c1] = _rotation_translation_chunk(seed, stream, start + c0, c1 - c0, trange, it)
+    return out
+
+
+def _rotation_translation_chunk(seed: int, stream: int, start: int, count: int, trange: float,
+                                items=None) -> np.ndarray:
+    u = rand_unit(seed, stream, start, count, 6, items).astype(np.float64)
     u1, u2, u3 = u[:, 0], u[:, 1], u[:, 2]
     a = np.sqrt(1.0 - u1)
     b = np.sqrt(u1)
@@ -217,33 +243,61 @@ def plain_scene(n: int = 100_000, seed: int = SEED_BASE + 0, num_statesets: int 
                  counts, [DEFAULT_CAMERA])
 
 
+class TreeSpec:
+    """A transform tree whose level sizes are fanouts[0], fanouts[0]*fanouts[1],
+    ...; the leaf level (the drawables) is ordered by StateSet, tree order
+    inside a StateSet.  Levels can be produced chunk by chunk (each matrix is
+    a function of its tree index), so large scenes stream to the device
+    without whole-level host arrays."""
+
+    def __init__(self, fanouts=(1000, 100, 100), seed: int = SEED_BASE + 1, num_statesets: int = 64,
+                 lod_count: int = 3, num_meshes: int = 64, trange: float = 50.0, name: str = "tree"):
+        self.fanouts, self.seed, self.trange, self.name = tuple(fanouts), seed, trange, name
+        self.meshes = make_meshes(seed, num_meshes, lod_count)
+        self.sizes = [int(x) for x in np.cumprod(fanouts)]
+        leaves = self.sizes[-1]
+        self.order, self.stateset_counts = _group_by_stateset(seed, 5, leaves, num_statesets)
+        self.mesh_index = np.ascontiguousarray(
+            rand_int(seed, 3, 0, leaves, 0, num_meshes - 1).astype(np.uint32)[self.order])
+        self.cameras = [DEFAULT_CAMERA]
+
+    @property
+    def num_drawables(self) -> int:
+        return self.sizes[-1]
+
+    def level_chunk(self, d: int, c0: int, c1: int):
+        last = d == len(self.fanouts) - 1
+        items = self.order[c0:c1] if last else None
+        local = rotation_translation(self.seed, 10 + d, c0, c1 - c0, self.trange, items)
+        if d == 0:
+            parent = None
+        else:
+            tree_index = items.astype(np.uint64) if last else np.arange(c0, c1, dtype=np.uint64)
+            parent = (tree_index // np.uint64(self.fanouts[d])).astype(np.uint32)
+        return local, parent
+
+    def to_scene(self) -> Scene:
+        levels = []
+        for d, n in enumerate(self.sizes):
+            local, parent = self.level_chunk(d, 0, n)
+            levels.append((local, parent))
+        return Scene(self.name, levels, self.mesh_index, self.meshes, self.stateset_counts, list(self.cameras))
+
+
 def tree_scene(fanouts=(1000, 100, 100), seed: int = SEED_BASE + 1, num_statesets: int = 64,
                lod_count: int = 3, num_meshes: int = 64, trange: float = 50.0) -> Scene:
-    """Tree with level sizes fanouts[0], fanouts[0]*fanouts[1], ...; the leaf
-    level is reordered into StateSet groups (tree order inside a group)."""
-    meshes = make_meshes(seed, num_meshes, lod_count)
-    levels = []
-    n_prev = None
-    n = 1
-    for d, f in enumerate(fanouts):
-        n = n * f
-        local = rotation_translation(seed, 10 + d, 0, n, trange)
-        parent = None if d == 0 else (np.arange(n, dtype=np.uint64) // f).astype(np.uint32)
-        levels.append([local, parent])
-        n_prev = n
-    leaves = n_prev
-    mesh_index = rand_int(seed, 3, 0, leaves, 0, num_meshes - 1).astype(np.uint32)
-    order, counts = _group_by_stateset(seed, 5, leaves, num_statesets)
-    local, parent = levels[-1]
-    levels[-1] = [np.ascontiguousarray(local[order]), None if parent is None else np.ascontiguousarray(parent[order])]
-    mesh_index = np.ascontiguousarray(mesh_index[order])
-    return Scene("tree", [tuple(l) for l in levels], mesh_index, meshes, counts, [DEFAULT_CAMERA])
+    """Materialised TreeSpec (host arrays for every level)."""
+    return TreeSpec(fanouts, seed, num_statesets, lod_count, num_meshes, trange).to_scene()
+
+
+def full10m_spec(scale: float = 1.0) -> TreeSpec:
+    """The full10m workload; scale < 1 shrinks the root count."""
+    roots = max(1, int(round(1000 * scale)))
+    return TreeSpec((roots, 100, 100), seed=SEED_BASE + 1, name="full10m")
 
 
 def full10m_scene(scale: float = 1.0) -> Scene:
-    """The full10m workload; scale < 1 shrinks the root count (for tests)."""
-    roots = max(1, int(round(1000 * scale)))
-    return tree_scene((roots, 100, 100), seed=SEED_BASE + 1)
+    return full10m_spec(scale).to_scene()
 
 
 def batch_cameras(count: int = 256) -> list:
