@@ -126,7 +126,7 @@ def main():
             if last:
                 t_leaf.end(s)
         t_cull.begin(s)
-        batch1.run(s)
+        batch1.run_view(s)
         t_cull.end(s)
 
     for _ in range(args.warmup):
@@ -217,7 +217,7 @@ def bench_batched(args, ds, config, stream, lib, N, cull, scenes, device):
 
     def run_plain():
         for j, dj in plain_jobs:
-            N.check(lib.cadrCullBatch(ctypes.byref(config), dj.address, N.ptr(j), 1, ws.handle, s))
+            N.check(lib.cadrCullView(ctypes.byref(config), dj.address, N.ptr(j), ws.handle, s))
 
     for _ in range(2):
         run_plain()

@@ -32,7 +32,7 @@ VIEW_DTYPE = np.dtype([("viewRow", "<f4", (3, 4)), ("nearZ", "<f4"), ("farZ", "<
                        ("sideX", "<f4", (2,)), ("sideY", "<f4", (2,)), ("lodScale", "<f4"),
                        ("lodThreshold", "<f4", (3,)), ("reserved", "<f4", (2,))])
 TILE_DTYPE = np.dtype([("start", "<u4"), ("count", "<u4"), ("stateSetBase", "<u4"),
-                       ("stateSetAndFlags", "<u4")])
+                       ("stateSetAndFlags", "<u4"), ("chainIndex", "<u4"), ("reserved", "<u4", (3,))])
 JOB_DTYPE = np.dtype([("spheres", "<u8"), ("meshIndex", "<u8"), ("meshes", "<u8"), ("tiles", "<u8"),
                       ("views", "<u8"), ("commands", "<u8"), ("counts", "<u8"),
                       ("numDrawables", "<u4"), ("numTiles", "<u4"), ("numStateSets", "<u4"),
@@ -49,7 +49,7 @@ LEVEL_DTYPE = np.dtype([("localMatrices", "<u8"), ("parentIndex", "<u8"), ("pare
                         ("parentCount", "<u4")])
 
 assert DRAWABLE_DTYPE.itemsize == 48 and MESH_DTYPE.itemsize == 64 and VIEW_DTYPE.itemsize == 96
-assert TILE_DTYPE.itemsize == 16 and JOB_DTYPE.itemsize == 80 and CMD_DTYPE.itemsize == 20
+assert TILE_DTYPE.itemsize == 32 and JOB_DTYPE.itemsize == 80 and CMD_DTYPE.itemsize == 20
 assert LEVEL_DTYPE.itemsize == 72
 
 TILE_FIRST = 0x80000000

@@ -7,30 +7,31 @@
 // drawables of StateSet k compacted to the front of its range plus a draw
 // count per StateSet (the operands of vkCmdDrawIndexedIndirectCount).
 //
-// One kernel serves the plain single-view pass, the many-view batch and the
-// uneven many-scene batch:
-//   - Work items are tiles of at most blockSize*itemsPerThread drawables that
-//     never cross a StateSet boundary, over all jobs of the launch.  A
-//     persistent grid takes tiles from an atomic ticket in increasing order,
-//     so a tile only ever waits on tiles that are already running.
-//   - A tile's spheres and mesh indices are read once from HBM (16 + 4 B per
-//     drawable, coalesced: wave w owns a contiguous run of 64*itemsPerThread
-//     drawables, item j of lane l is drawable w*64*itemsPerThread + j*64 + l),
-//     staged in LDS, and then tested against every view of the job, 64 views
-//     at a time.
-//   - Visibility of (view, drawable) becomes one bit of a wave64 ballot kept
-//     in LDS; the per-view tile count is a scalar popcount.
-//   - Wave 0 then runs the decoupled look-back of all 64 views at once, lane
-//     v walking the status chain of view v: status words carry {epoch, state
-//     (aggregate / inclusive prefix), value}, the first tile of a StateSet
-//     publishes its inclusive prefix at once, so chains never cross
-//     StateSets.
-//   - Writes are compacted per wave through an LDS index list: lane l emits
-//     the l-th visible drawable of the wave, so a wave stores a contiguous run
-//     of 20-byte commands; the LOD is evaluated for visible drawables only.
-//   - A conservative bounding sphere of each tile, tested per view with a
-//     margin far above FP32 rounding, lets whole (tile, view) pairs skip the
-//     per-drawable tests (all outside or all inside).
+// Work is split into tiles of at most blockSize*itemsPerThread drawables that
+// never cross a StateSet boundary; tiles are taken from an atomic ticket in
+// array order, so a tile only ever waits on tiles that already run.  Within a
+// tile, wave w owns a contiguous run of 64*itemsPerThread drawables (item j of
+// lane l is drawable w*64*itemsPerThread + j*64 + l): loads are coalesced and
+// the visibility of 64 drawables is one wave64 ballot.  Offsets come from a
+// single-pass decoupled look-back: status words carry {epoch, state
+// (aggregate / inclusive prefix), value}; the first tile of a StateSet
+// publishes its inclusive prefix at once, and tiles are issued round-robin
+// over StateSets, so the walks stay short.  Status words are tagged with a
+// launch epoch and never cleared.
+//
+// Two kernels:
+//   cullViewKernel - one view: one workgroup per tile, visibility kept in
+//     registers, a 64-lane parallel look-back in wave 0, direct 20-byte
+//     writes.  It resets its ticket and advances the epoch itself (the last
+//     workgroup to finish), so a view costs one launch.
+//   cullKernel - many views and many scenes (jobs) in one persistent launch:
+//     a tile is read from HBM once, staged in LDS and tested against all views
+//     of its job, 64 views at a time; lane v of wave 0 walks the chain of view
+//     v; writes are compacted per wave through an LDS index list, so lane l
+//     stores the l-th visible drawable.  A conservative bounding sphere of the
+//     tile, classified per view with lanes as views, lets whole (tile, view)
+//     pairs skip the per-drawable test when the tile is wholly outside or
+//     wholly inside the frustum.
 #include "internal.h"
 
 #include <algorithm>
@@ -39,9 +40,11 @@
 namespace cadr {
 
 constexpr uint32_t kViewGroup = 64;   // views whose look-backs share one wave
-constexpr uint32_t kCtlTicket = 0;    // ctl word: next tile ticket
-constexpr uint32_t kCtlEpoch = 32;    // ctl word: launch epoch (own 128-B line)
-constexpr uint32_t kCtlWords = 64;
+constexpr uint32_t kCtlTicket = 0;    // ctl words, each on its own 128-byte line
+constexpr uint32_t kCtlDone = 32;
+constexpr uint32_t kCtlEpoch = 64;
+constexpr uint32_t kCtlWords = 96;
+constexpr uint32_t kLdsMeshes = 64;   // mesh tables up to this size are staged in LDS
 
 constexpr uint32_t kStateAggregate = 1;
 constexpr uint32_t kStatePrefix = 2;
@@ -58,6 +61,31 @@ struct CullParams {
 	uint32_t flags;
 };
 
+struct TileDesc {
+	uint32_t start, count, segBase, k, chain;
+	bool first, last;
+};
+
+__device__ __forceinline__ TileDesc loadTile(const cadrCullJob& J, uint32_t t, uint32_t tileSize)
+{
+	const uint4* p = reinterpret_cast<const uint4*>(J.tiles) + 2 * uint64_t(t);
+	const uint4 a = p[0];
+	const uint4 b = p[1];
+	TileDesc d;
+	d.start = a.x;
+	d.count = a.y;
+	d.segBase = a.z;
+	d.k = a.w & CADR_TILE_STATESET_MASK;
+	d.first = (a.w & CADR_TILE_FIRST) != 0u;
+	d.last = (a.w & CADR_TILE_LAST) != 0u;
+	d.chain = b.x;
+	if (d.chain >= J.numTiles) d.chain = t;  // malformed list: keep status accesses in range
+	if (d.start >= J.numDrawables) d.count = 0;
+	else if (d.count > J.numDrawables - d.start) d.count = J.numDrawables - d.start;
+	if (d.count > tileSize) d.count = tileSize;
+	return d;
+}
+
 // Per-view constants, loaded with scalar loads (the view index is uniform).
 struct ViewC {
 	float4 r0, r1, r2;
@@ -73,7 +101,7 @@ __device__ __forceinline__ ViewC loadView(const cadrView* __restrict__ views, ui
 	c.r2 = p[2];
 	const float4 a = p[3];  // nearZ, farZ, sideX[0], sideX[1]
 	const float4 b = p[4];  // sideY[0], sideY[1], lodScale, lodThreshold[0]
-	const float4 d = p[5];  // lodThreshold[1], lodThreshold[2], reserved...
+	const float4 d = p[5];  // lodThreshold[1], lodThreshold[2], reserved
 	c.nearZ = a.x; c.farZ = a.y; c.sx0 = a.z; c.sx1 = a.w;
 	c.sy0 = b.x; c.sy1 = b.y; c.lodScale = b.z; c.th0 = b.w;
 	c.th1 = d.x; c.th2 = d.y;
@@ -99,11 +127,10 @@ __device__ __forceinline__ bool sphereVisible(float4 s, const ViewC& v)
 	return nearOk & farOk & xOk & yOk;
 }
 
-__device__ __forceinline__ uint32_t selectLod(float4 s, const ViewC& v, uint32_t lodCount)
+__device__ __forceinline__ uint32_t lodFromDepth(float r, float vz, const ViewC& v, uint32_t lodCount)
 {
-	const float vz = viewCoord(v.r2, s);
-	const float p = s.w * v.lodScale;
-	uint32_t lod = uint32_t(p < v.th0 * vz) + uint32_t(p < v.th1 * vz) + uint32_t(p < v.th2 * vz);
+	const float p = r * v.lodScale;
+	const uint32_t lod = uint32_t(p < v.th0 * vz) + uint32_t(p < v.th1 * vz) + uint32_t(p < v.th2 * vz);
 	const uint32_t maxLod = (lodCount >= 1u ? (lodCount <= 4u ? lodCount : 4u) : 1u) - 1u;
 	return lod < maxLod ? lod : maxLod;
 }
@@ -169,20 +196,30 @@ __device__ __forceinline__ void publish(uint64_t* p, uint64_t v)
 	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive prefix of tile t in one chain: walk back over the predecessors,
-// adding aggregates until an inclusive prefix (every StateSet's first tile
-// publishes one at once) is found.  Each predecessor holds an earlier ticket
-// and is running or done, so every wait ends.
-__device__ uint32_t lookBack(const uint64_t* chain, uint32_t t, uint32_t epoch)
+__device__ __forceinline__ uint64_t readStatus(const uint64_t* p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool statusReady(uint64_t s, uint32_t epoch)
+{
+	return uint32_t(s >> 32) == epoch && (uint32_t(s) >> 30) != 0u;
+}
+
+// Exclusive prefix of chain position c, one chain per lane: walk back over the
+// predecessors adding aggregates until an inclusive prefix is found.  Each
+// predecessor holds an earlier ticket and is running or done, so every wait
+// ends; position 0 ends the walk in any case.
+__device__ uint32_t lookBackSerial(const uint64_t* chain, uint32_t c, uint32_t epoch)
 {
 	uint32_t excl = 0;
-	uint32_t p = t;
+	uint32_t p = c;
 	while (p > 0) {
 		--p;
 		uint64_t s;
 		for (;;) {
-			s = __hip_atomic_load(chain + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			if (uint32_t(s >> 32) == epoch && (uint32_t(s) >> 30) != 0u) break;
+			s = readStatus(chain + p);
+			if (statusReady(s, epoch)) break;
 			__builtin_amdgcn_s_sleep(1);
 		}
 		excl += uint32_t(s) & kValueMask;
@@ -190,6 +227,165 @@ __device__ uint32_t lookBack(const uint64_t* chain, uint32_t t, uint32_t epoch)
 	}
 	return excl;
 }
+
+// Exclusive prefix of chain position c with the whole wave: lane l reads
+// position p-1-l, the walk stops at the nearest inclusive prefix, and 64
+// predecessors cost one round trip.  Returns the same value in every lane.
+__device__ uint32_t lookBackWave(const uint64_t* chain, uint32_t c, uint32_t epoch, uint32_t lane)
+{
+	uint32_t excl = 0;
+	int64_t p = c;
+	while (p > 0) {
+		const int64_t idx = p - 1 - int64_t(lane);
+		const uint64_t s = idx >= 0 ? readStatus(chain + idx) : packStatus(epoch, kStatePrefix, 0);
+		const bool ready = statusReady(s, epoch);
+		const uint64_t pmask = __ballot(ready && (uint32_t(s) >> 30) == kStatePrefix);
+		const uint64_t nmask = __ballot(!ready);
+		const uint32_t f = pmask ? uint32_t(__builtin_ctzll(pmask)) : 64u;
+		const uint64_t need = f >= 63u ? ~0ull : ((2ull << f) - 1ull);
+		if (nmask & need) {
+			__builtin_amdgcn_s_sleep(1);
+			continue;
+		}
+		uint32_t v = lane <= f ? (uint32_t(s) & kValueMask) : 0u;
+#pragma unroll
+		for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+		excl += v;
+		if (f < 64u) break;
+		p -= 64;
+	}
+	return excl;
+}
+
+__device__ __forceinline__ void writeCommand(uint32_t* o, const cadrMesh& mesh, uint32_t lod, uint32_t drawable)
+{
+	__builtin_nontemporal_store(mesh.indexCount[lod], o + 0);
+	__builtin_nontemporal_store(1u, o + 1);
+	__builtin_nontemporal_store(mesh.firstIndex[lod], o + 2);
+	__builtin_nontemporal_store(uint32_t(mesh.vertexOffset), o + 3);
+	__builtin_nontemporal_store(drawable, o + 4);
+}
+
+// Mesh table of a job staged in LDS when it is small (one 16-byte word per
+// thread); returns whether the LDS copy is in use.
+template <int BLOCK>
+__device__ __forceinline__ bool stageMeshes(cadrMesh* sMeshes, const cadrCullJob& J, uint32_t tid)
+{
+	if (J.numMeshes > kLdsMeshes) return false;
+	const float4* src = reinterpret_cast<const float4*>(J.meshes);
+	float4* dst = reinterpret_cast<float4*>(sMeshes);
+	for (uint32_t i = tid; i < J.numMeshes * 4u; i += BLOCK) dst[i] = src[i];
+	return true;
+}
+
+// ---------------------------------------------------------------- one view
+
+template <int ITEMS>
+__device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCullJob& J, uint32_t t, uint32_t epoch,
+                                             const cadrMesh* meshes, uint32_t (*sCnt)[ITEMS], uint32_t* sExclP,
+                                             uint32_t tid, uint32_t lane, uint32_t wave)
+{
+	constexpr int BLOCK = 256, WAVES = 4, WAVE_ITEMS = 64 * ITEMS;
+	const TileDesc T = loadTile(J, t, BLOCK * ITEMS);
+	const ViewC V = loadView(reinterpret_cast<const cadrView*>(J.views), 0);
+	const float4* __restrict__ spheres = reinterpret_cast<const float4*>(J.spheres);
+	const uint32_t* __restrict__ meshIndex = reinterpret_cast<const uint32_t*>(J.meshIndex);
+
+	float vz[ITEMS], rad[ITEMS];
+	uint32_t mi[ITEMS];
+	uint64_t mask[ITEMS];
+#pragma unroll
+	for (int j = 0; j < ITEMS; j++) {
+		const uint32_t e = wave * WAVE_ITEMS + j * 64 + lane;
+		bool vis = false;
+		vz[j] = 0.f;
+		rad[j] = 0.f;
+		mi[j] = 0;
+		if (e < T.count) {
+			const float4 s = spheres[T.start + e];
+			const uint32_t m = meshIndex[T.start + e];
+			vis = m < J.numMeshes && sphereVisible(s, V);
+			vz[j] = viewCoord(V.r2, s);
+			rad[j] = s.w;
+			mi[j] = m;
+		}
+		mask[j] = __ballot(vis);
+		if (lane == 0) sCnt[wave][j] = uint32_t(__builtin_popcountll(mask[j]));
+	}
+	__syncthreads();
+
+	if (wave == 0) {
+		uint32_t total = 0;
+		for (int q = 0; q < WAVES; q++)
+			for (int j = 0; j < ITEMS; j++) total += sCnt[q][j];
+		uint64_t* chain = P.status;
+		uint32_t excl = 0;
+		if (T.first) {
+			if (lane == 0) publish(chain + T.chain, packStatus(epoch, kStatePrefix, total));
+		} else {
+			if (lane == 0) publish(chain + T.chain, packStatus(epoch, kStateAggregate, total));
+			excl = lookBackWave(chain, T.chain, epoch, lane);
+			if (lane == 0) publish(chain + T.chain, packStatus(epoch, kStatePrefix, excl + total));
+		}
+		if (lane == 0) {
+			*sExclP = excl;
+			if (T.last && T.k < J.numStateSets) reinterpret_cast<uint32_t*>(J.counts)[T.k] = excl + total;
+		}
+	}
+	__syncthreads();
+
+	uint32_t pos = T.segBase + *sExclP;
+	for (uint32_t q = 0; q < wave; q++)
+		for (int j = 0; j < ITEMS; j++) pos += sCnt[q][j];
+	uint32_t* out = reinterpret_cast<uint32_t*>(J.commands);
+#pragma unroll
+	for (int j = 0; j < ITEMS; j++) {
+		if ((mask[j] >> lane) & 1ull) {
+			const uint32_t at = pos + maskRank(mask[j]);
+			if (at < J.commandStride) {
+				const cadrMesh& mesh = meshes[mi[j]];
+				const uint32_t lod = lodFromDepth(rad[j], vz[j], V, mesh.lodCount);
+				writeCommand(out + uint64_t(at) * 5ull, mesh, lod, T.start + wave * WAVE_ITEMS + j * 64 + lane);
+			}
+		}
+		pos += uint32_t(__builtin_popcountll(mask[j]));
+	}
+}
+
+
+template <int ITEMS>
+__global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
+{
+	constexpr int BLOCK = 256, WAVES = 4;
+	__shared__ uint32_t sTicket;
+	__shared__ uint32_t sCnt[WAVES][ITEMS];
+	__shared__ uint32_t sExcl;
+	__shared__ cadrMesh sMeshes[kLdsMeshes];
+
+	const uint32_t tid = threadIdx.x, lane = laneId(), wave = tid / 64;
+	const uint32_t epoch = __hip_atomic_load(&P.ctl[kCtlEpoch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	if (tid == 0) sTicket = atomicAdd(&P.ctl[kCtlTicket], 1u);
+	const cadrCullJob J = P.jobs[0];
+	const bool ldsMeshes = stageMeshes<BLOCK>(sMeshes, J, tid);
+	__syncthreads();
+	const uint32_t t = uniform(sTicket);
+	if (t < P.totalWork) cullViewTile<ITEMS>(P, J, t, epoch, ldsMeshes ? sMeshes : reinterpret_cast<const cadrMesh*>(J.meshes),
+	                                         sCnt, &sExcl, tid, lane, wave);
+
+	// the last workgroup to finish resets the ticket and starts a new epoch
+	__syncthreads();
+	if (tid == 0) {
+		const uint32_t d = atomicAdd(&P.ctl[kCtlDone], 1u);
+		if (d == gridDim.x - 1) {
+			atomicExch(&P.ctl[kCtlTicket], 0u);
+			atomicExch(&P.ctl[kCtlDone], 0u);
+			const uint32_t e = epoch + 1u;
+			atomicExch(&P.ctl[kCtlEpoch], e == 0u ? 1u : e);
+		}
+	}
+}
+
+// ------------------------------------------------------- many views, jobs
 
 template <int BLOCK, int ITEMS>
 struct CullShared {
@@ -201,6 +397,7 @@ struct CullShared {
 	uint32_t cnt[kViewGroup][WAVES];
 	uint32_t excl[kViewGroup];
 	uint16_t list[WAVES][64 * ITEMS];
+	cadrMesh meshes[kLdsMeshes];
 	float red[WAVES][6];
 	uint32_t bad[WAVES];
 	uint32_t ticket;
@@ -217,7 +414,7 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 	const uint32_t tid = threadIdx.x;
 	const uint32_t lane = laneId();
 	const uint32_t wave = tid / 64;
-	const uint32_t epoch = P.ctl[kCtlEpoch];
+	const uint32_t epoch = __hip_atomic_load(&P.ctl[kCtlEpoch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	uint32_t job = 0;
 
 	for (;;) {
@@ -237,42 +434,38 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		}
 		const cadrCullJob J = P.jobs[job];
 		const uint32_t t = w - P.firstWork[job];
-		const uint4 tileWord = reinterpret_cast<const uint4*>(J.tiles)[t];
-		uint32_t start = tileWord.x, count = tileWord.y;
-		const uint32_t segBase = tileWord.z;
-		const uint32_t k = tileWord.w & CADR_TILE_STATESET_MASK;
-		const bool first = (tileWord.w & CADR_TILE_FIRST) != 0u;
-		const bool last = (tileWord.w & CADR_TILE_LAST) != 0u;
-		if (start >= J.numDrawables) count = 0;
-		else if (count > J.numDrawables - start) count = J.numDrawables - start;
-		if (count > uint32_t(S::TILE)) count = S::TILE;
+		const TileDesc T = loadTile(J, t, S::TILE);
+		const bool ldsMeshes = stageMeshes<BLOCK>(sh.meshes, J, tid);
 
 		// load the tile: wave-contiguous runs, staged in LDS, bounds for the tile sphere
 		const float4* __restrict__ spheres = reinterpret_cast<const float4*>(J.spheres);
 		const uint32_t* __restrict__ meshIndex = reinterpret_cast<const uint32_t*>(J.meshIndex);
 		float4 sp[ITEMS];
-		bool nonFinite = false;
+		bool irregular = false;
 		float mn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
 		float mx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 #pragma unroll
 		for (int j = 0; j < ITEMS; j++) {
 			const uint32_t e = wave * WAVE_ITEMS + j * 64 + lane;
 			float4 s = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
-			uint32_t m = 0xffffffffu;
-			if (e < count) {
-				s = spheres[start + e];
-				m = meshIndex[start + e];
-				if (m >= J.numMeshes) s.w = -__builtin_inff();  // invalid mesh: never visible
+			uint32_t m = 0;
+			if (e < T.count) {
+				s = spheres[T.start + e];
+				m = meshIndex[T.start + e];
+				if (m >= J.numMeshes) {
+					s.w = -__builtin_inff();  // invalid mesh: never visible
+					m = 0;
+				}
 			}
 			sp[j] = s;
 			sh.sph[e] = s;
 			sh.mesh[e] = m;
 			// a sphere other than a finite one with r >= 0 or the empty sphere
 			// (r = -inf, never visible) disables the tile sphere for this tile
-			if (e < count && s.w != -__builtin_inff() &&
+			if (e < T.count && s.w != -__builtin_inff() &&
 			    !(__builtin_isfinite(s.x) && __builtin_isfinite(s.y) && __builtin_isfinite(s.z) &&
 			      __builtin_isfinite(s.w) && s.w >= 0.f))
-				nonFinite = true;
+				irregular = true;
 			if (s.w >= 0.f) {
 				mn[0] = fminf(mn[0], s.x - s.w); mx[0] = fmaxf(mx[0], s.x + s.w);
 				mn[1] = fminf(mn[1], s.y - s.w); mx[1] = fmaxf(mx[1], s.y + s.w);
@@ -280,7 +473,6 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			}
 		}
 		const bool useTileSphere = (P.flags & CADR_CULL_NO_TILE_SPHERES) == 0u;
-		float4 tileSphere = make_float4(0.f, 0.f, 0.f, -1.f);
 		if (useTileSphere) {
 #pragma unroll
 			for (int off = 32; off >= 1; off >>= 1) {
@@ -290,64 +482,67 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 					mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
 				}
 			}
-			const bool waveNonFinite = __ballot(nonFinite) != 0ull;
+			const bool waveIrregular = __ballot(irregular) != 0ull;
 			if (lane == 0) {
-				sh.bad[wave] = waveNonFinite ? 1u : 0u;
+				sh.bad[wave] = waveIrregular ? 1u : 0u;
 				sh.red[wave][0] = mn[0]; sh.red[wave][1] = mn[1]; sh.red[wave][2] = mn[2];
 				sh.red[wave][3] = mx[0]; sh.red[wave][4] = mx[1]; sh.red[wave][5] = mx[2];
 			}
 		}
 		__syncthreads();
 		bool tileUsable = false;
+		float4 tileSphere = make_float4(0.f, 0.f, 0.f, -1.f);
 		if (useTileSphere) {
+			uint32_t anyBad = 0;
 #pragma unroll
 			for (int q = 0; q < WAVES; q++) {
+				anyBad |= sh.bad[q];
 				for (int a = 0; a < 3; a++) {
 					mn[a] = fminf(mn[a], sh.red[q][a]);
 					mx[a] = fmaxf(mx[a], sh.red[q][3 + a]);
 				}
 			}
-			// finite bounds only (an empty or NaN tile keeps per-drawable tests)
 			const float hx = (mx[0] - mn[0]) * 0.5f, hy = (mx[1] - mn[1]) * 0.5f, hz = (mx[2] - mn[2]) * 0.5f;
 			const float cx = (mn[0] + mx[0]) * 0.5f, cy = (mn[1] + mx[1]) * 0.5f, cz = (mn[2] + mx[2]) * 0.5f;
 			const float R = sqrtf(hx * hx + hy * hy + hz * hz);
-			uint32_t anyBad = 0;
-#pragma unroll
-			for (int q = 0; q < WAVES; q++) anyBad |= sh.bad[q];
 			tileUsable = anyBad == 0u && __builtin_isfinite(R) && __builtin_isfinite(cx) &&
 			             __builtin_isfinite(cy) && __builtin_isfinite(cz);
 			// inflate against rounding of the centre and radius above
 			const float slack = (fabsf(cx) + fabsf(cy) + fabsf(cz) + R) * (1.0f / 65536.0f);
 			tileSphere = make_float4(cx, cy, cz, R + slack);
 		}
-		const bool emptyTile = count == 0;
+		const bool emptyTile = T.count == 0;
 
 		const cadrView* __restrict__ views = reinterpret_cast<const cadrView*>(J.views);
 		uint64_t* chain0 = P.status + P.statusBase[job];
-		const cadrMesh* __restrict__ meshes = reinterpret_cast<const cadrMesh*>(J.meshes);
+		const cadrMesh* meshes = ldsMeshes ? sh.meshes : reinterpret_cast<const cadrMesh*>(J.meshes);
 
 		for (uint32_t g0 = 0; g0 < J.numViews; g0 += kViewGroup) {
 			const uint32_t vg = uniform(min(kViewGroup, J.numViews - g0));
 
+			// tile classes of the group's views, lane v holding view g0+v
+			uint32_t cls = 0;
+			if (emptyTile) cls = 1;
+			else if (tileUsable && lane < vg) cls = classifyTile(tileSphere, loadView(views, g0 + lane));
+
 			// 1. visibility bits and per-wave counts
 			for (uint32_t v = 0; v < vg; v++) {
+				const uint32_t c = uniform(__builtin_amdgcn_readlane(cls, v));
+				if (c == 1u) {
+					if (lane == 0) sh.cnt[v][wave] = 0;
+					continue;
+				}
 				const ViewC V = loadView(views, g0 + v);
-				uint32_t cls = 0;
-				if (emptyTile) cls = 1;
-				else if (tileUsable) cls = uniform(classifyTile(tileSphere, V));
-				uint32_t c = 0;
+				uint32_t n = 0;
 #pragma unroll
 				for (int j = 0; j < ITEMS; j++) {
 					const uint32_t e = wave * WAVE_ITEMS + j * 64 + lane;
-					bool vis;
-					if (cls == 1u) vis = false;
-					else if (cls == 2u) vis = (e < count) && (sp[j].w >= 0.f);
-					else vis = (e < count) && sphereVisible(sp[j], V);
+					const bool vis = (e < T.count) && (c == 2u ? sp[j].w >= 0.f : sphereVisible(sp[j], V));
 					const uint64_t m = __ballot(vis);
 					if (lane == 0) sh.mask[v][wave][j] = m;
-					c += uint32_t(__builtin_popcountll(m));
+					n += uint32_t(__builtin_popcountll(m));
 				}
-				if (lane == 0) sh.cnt[v][wave] = c;
+				if (lane == 0) sh.cnt[v][wave] = n;
 			}
 			__syncthreads();
 
@@ -358,60 +553,67 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 				for (int q = 0; q < WAVES; q++) total += sh.cnt[lane][q];
 				uint64_t* chain = chain0 + uint64_t(g0 + lane) * J.numTiles;
 				uint32_t excl = 0;
-				if (first) {
-					publish(chain + t, packStatus(epoch, kStatePrefix, total));
+				if (T.first) {
+					publish(chain + T.chain, packStatus(epoch, kStatePrefix, total));
 				} else {
-					publish(chain + t, packStatus(epoch, kStateAggregate, total));
-					excl = lookBack(chain, t, epoch);
-					publish(chain + t, packStatus(epoch, kStatePrefix, excl + total));
+					publish(chain + T.chain, packStatus(epoch, kStateAggregate, total));
+					excl = lookBackSerial(chain, T.chain, epoch);
+					publish(chain + T.chain, packStatus(epoch, kStatePrefix, excl + total));
 				}
 				sh.excl[lane] = excl;
-				if (last && k < J.numStateSets)
-					reinterpret_cast<uint32_t*>(J.counts)[uint64_t(g0 + lane) * J.numStateSets + k] = excl + total;
+				if (T.last && T.k < J.numStateSets)
+					reinterpret_cast<uint32_t*>(J.counts)[uint64_t(g0 + lane) * J.numStateSets + T.k] = excl + total;
 			}
 			__syncthreads();
 
-			// 3. compacted writes
+			// 3. compacted writes: lane r of the wave stores its r-th visible drawable
 			for (uint32_t v = 0; v < vg; v++) {
+				const uint32_t run = uniform(sh.cnt[v][wave]);
+				if (run == 0) continue;
 				uint32_t wavePos = sh.excl[v];
 				for (uint32_t q = 0; q < wave; q++) wavePos += sh.cnt[v][q];
 				wavePos = uniform(wavePos);
-				uint32_t run = 0;
+				uint32_t r0 = 0;
 #pragma unroll
 				for (int j = 0; j < ITEMS; j++) {
 					const uint64_t m = sh.mask[v][wave][j];
-					if ((m >> lane) & 1ull) sh.list[wave][run + maskRank(m)] = uint16_t(j * 64 + lane);
-					run += uint32_t(__builtin_popcountll(m));
+					if ((m >> lane) & 1ull) sh.list[wave][r0 + maskRank(m)] = uint16_t(j * 64 + lane);
+					r0 += uint32_t(__builtin_popcountll(m));
 				}
-				run = uniform(run);
-				if (run == 0) continue;
 				waveSync();
 				const ViewC V = loadView(views, g0 + v);
 				uint32_t* out = reinterpret_cast<uint32_t*>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
 				for (uint32_t r = lane; r < run; r += 64) {
 					const uint32_t e = wave * WAVE_ITEMS + sh.list[wave][r];
-					const float4 s = sh.sph[e];
-					const uint32_t mi = sh.mesh[e];
-					const uint32_t pos = segBase + wavePos + r;
+					const uint32_t pos = T.segBase + wavePos + r;
 					if (pos >= J.commandStride) continue;
-					const cadrMesh* mesh = meshes + mi;
-					const uint32_t lod = selectLod(s, V, mesh->lodCount);
-					uint32_t* o = out + uint64_t(pos) * 5ull;
-					__builtin_nontemporal_store(mesh->indexCount[lod], o + 0);
-					__builtin_nontemporal_store(1u, o + 1);
-					__builtin_nontemporal_store(mesh->firstIndex[lod], o + 2);
-					__builtin_nontemporal_store(uint32_t(mesh->vertexOffset), o + 3);
-					__builtin_nontemporal_store(start + e, o + 4);
+					const float4 s = sh.sph[e];
+					const cadrMesh& mesh = meshes[sh.mesh[e]];
+					const uint32_t lod = lodFromDepth(s.w, viewCoord(V.r2, s), V, mesh.lodCount);
+					writeCommand(out + uint64_t(pos) * 5ull, mesh, lod, T.start + e);
 				}
 				waveSync();
 			}
 			__syncthreads();
 		}
 	}
+
+	// the last workgroup to leave resets the ticket and retires this launch's
+	// epoch, so the next launch on the workspace (of either kernel) never sees
+	// this launch's status words as its own
+	if (tid == 0) {
+		const uint32_t d = atomicAdd(&P.ctl[kCtlDone], 1u);
+		if (d == gridDim.x - 1) {
+			atomicExch(&P.ctl[kCtlTicket], 0u);
+			atomicExch(&P.ctl[kCtlDone], 0u);
+			const uint32_t e = epoch + 1u;
+			atomicExch(&P.ctl[kCtlEpoch], e == 0u ? 1u : e);
+		}
+	}
 }
 
-// Job offsets for the launch (prefix sums over jobs), ticket reset and a new
-// epoch; one block, before the cull kernel on the same stream.
+// Job offsets for a batched launch (prefix sums over jobs), ticket reset and
+// a new epoch; one block, before the cull kernel on the same stream.
 __global__ __launch_bounds__(256) void cullSetupKernel(const cadrCullJob* __restrict__ jobs, uint32_t numJobs,
                                                        uint32_t* __restrict__ firstWork,
                                                        uint64_t* __restrict__ statusBase, uint32_t* __restrict__ ctl)
@@ -451,6 +653,7 @@ __global__ __launch_bounds__(256) void cullSetupKernel(const cadrCullJob* __rest
 	if (tid == 0) {
 		firstWork[numJobs] = runT;
 		ctl[kCtlTicket] = 0;
+		ctl[kCtlDone] = 0;
 		uint32_t e = ctl[kCtlEpoch] + 1u;
 		ctl[kCtlEpoch] = e == 0u ? 1u : e;
 	}
@@ -476,13 +679,15 @@ using KernelFn = void (*)(CullParams);
 
 struct KernelEntry {
 	uint32_t block, items;
-	KernelFn fn;
+	KernelFn batch;   // many views / jobs
+	KernelFn view;    // one view (256-thread blocks only), or nullptr
 };
 
 const KernelEntry kKernels[] = {
-    {256, 2, cullKernel<256, 2>},  {256, 4, cullKernel<256, 4>}, {256, 8, cullKernel<256, 8>},
-    {512, 2, cullKernel<512, 2>},  {512, 4, cullKernel<512, 4>}, {128, 8, cullKernel<128, 8>},
-    {128, 16, cullKernel<128, 16>},
+    {256, 2, cullKernel<256, 2>, cullViewKernel<2>},   {256, 4, cullKernel<256, 4>, cullViewKernel<4>},
+    {256, 8, cullKernel<256, 8>, cullViewKernel<8>},   {256, 16, cullKernel<256, 16>, cullViewKernel<16>},
+    {512, 2, cullKernel<512, 2>, nullptr},             {512, 4, cullKernel<512, 4>, nullptr},
+    {128, 8, cullKernel<128, 8>, nullptr},             {128, 16, cullKernel<128, 16>, nullptr},
 };
 
 const KernelEntry* findKernel(const cadrCullConfig* c)
@@ -493,6 +698,18 @@ const KernelEntry* findKernel(const cadrCullConfig* c)
 	return nullptr;
 }
 
+int checkJob(const cadrCullJob& J)
+{
+	CADR_REQUIRE(J.numTiles == 0 || J.tiles, "cull: job without tiles");
+	CADR_REQUIRE(J.numTiles == 0 || (J.spheres && J.meshIndex && J.meshes && J.views && J.commands && J.counts),
+	             "cull: job with a null array");
+	CADR_REQUIRE(J.spheres % 16 == 0 && J.meshes % 16 == 0 && J.views % 16 == 0 && J.tiles % 16 == 0 &&
+	                 J.meshIndex % 4 == 0 && J.commands % 4 == 0 && J.counts % 4 == 0,
+	             "cull: misaligned job array (spheres, meshes, views, tiles need 16 bytes)");
+	CADR_REQUIRE(J.commandStride >= J.numDrawables, "cull: commandStride < numDrawables");
+	return CADR_SUCCESS;
+}
+
 }  // namespace
 
 CADR_API uint32_t cadrCullTileSize(const cadrCullConfig* config)
@@ -501,9 +718,6 @@ CADR_API uint32_t cadrCullTileSize(const cadrCullConfig* config)
 	return k ? k->block * k->items : 0u;
 }
 
-CADR_API int cadrMakeView(const float eye[3], const float dir[3], const float up[3], float fovy, float aspect,
-                          float zNear, float zFar, float viewportHeight, const float* lodThresholds, cadrView* out);
-
 CADR_API int cadrBuildCullTiles(const uint32_t* stateSetCounts, uint32_t numStateSets, uint32_t tileSize,
                                 cadrCullTile* tiles, uint32_t capacity, uint32_t* numTiles)
 {
@@ -511,27 +725,42 @@ CADR_API int cadrBuildCullTiles(const uint32_t* stateSetCounts, uint32_t numStat
 	CADR_REQUIRE(tileSize > 0, "cadrBuildCullTiles: tileSize is 0");
 	CADR_REQUIRE(numStateSets == 0 || stateSetCounts, "cadrBuildCullTiles: stateSetCounts is NULL");
 	CADR_REQUIRE(numStateSets <= CADR_TILE_STATESET_MASK, "cadrBuildCullTiles: too many StateSets");
-	uint64_t base = 0, n = 0;
+	std::vector<uint64_t> base(numStateSets + 1, 0), firstTile(numStateSets + 1, 0);
+	uint64_t maxTiles = 0;
 	for (uint32_t k = 0; k < numStateSets; k++) {
 		const uint64_t c = stateSetCounts[k];
 		const uint64_t nt = c == 0 ? 1 : (c + tileSize - 1) / tileSize;
-		for (uint64_t q = 0; q < nt; q++) {
-			if (tiles) {
-				if (n >= capacity) return setError(CADR_ERROR_INVALID_VALUE, "cadrBuildCullTiles: capacity too small");
-				const uint64_t s = base + q * tileSize;
-				const uint64_t cnt = c == 0 ? 0 : std::min<uint64_t>(tileSize, c - q * tileSize);
-				uint32_t flags = k;
-				if (q == 0) flags |= CADR_TILE_FIRST;
-				if (q + 1 == nt) flags |= CADR_TILE_LAST;
-				tiles[n] = cadrCullTile{uint32_t(s), uint32_t(cnt), uint32_t(base), flags};
-			}
-			n++;
-		}
-		base += c;
-		if (base > 0xffffffffull) return setError(CADR_ERROR_INVALID_VALUE, "cadrBuildCullTiles: more than 2^32 drawables");
+		base[k + 1] = base[k] + c;
+		firstTile[k + 1] = firstTile[k] + nt;
+		maxTiles = std::max(maxTiles, nt);
 	}
+	if (base[numStateSets] > 0xffffffffull)
+		return setError(CADR_ERROR_INVALID_VALUE, "cadrBuildCullTiles: more than 2^32 drawables");
+	const uint64_t n = firstTile[numStateSets];
 	if (n > 0xffffffffull) return setError(CADR_ERROR_INVALID_VALUE, "cadrBuildCullTiles: too many tiles");
 	*numTiles = uint32_t(n);
+	if (!tiles) return ok();
+	if (n > capacity) return setError(CADR_ERROR_INVALID_VALUE, "cadrBuildCullTiles: capacity too small");
+	// round-robin over StateSets: the q-th tile of every StateSet, then the (q+1)-th
+	uint64_t out = 0;
+	for (uint64_t q = 0; q < maxTiles; q++) {
+		for (uint32_t k = 0; k < numStateSets; k++) {
+			const uint64_t c = stateSetCounts[k];
+			const uint64_t nt = firstTile[k + 1] - firstTile[k];
+			if (q >= nt) continue;
+			const uint64_t cnt = c == 0 ? 0 : std::min<uint64_t>(tileSize, c - q * tileSize);
+			uint32_t flags = k;
+			if (q == 0) flags |= CADR_TILE_FIRST;
+			if (q + 1 == nt) flags |= CADR_TILE_LAST;
+			cadrCullTile& t = tiles[out++];
+			t.start = uint32_t(base[k] + q * tileSize);
+			t.count = uint32_t(cnt);
+			t.stateSetBase = uint32_t(base[k]);
+			t.stateSetAndFlags = flags;
+			t.chainIndex = uint32_t(firstTile[k] + q);
+			t.reserved[0] = t.reserved[1] = t.reserved[2] = 0;
+		}
+	}
 	return ok();
 }
 
@@ -582,31 +811,23 @@ CADR_API int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, co
 	CADR_REQUIRE(deviceJobs && hostJobs, "cadrCullBatch: null job array");
 	CADR_REQUIRE(deviceJobs % 16 == 0, "cadrCullBatch: deviceJobs must be 16-byte aligned");
 	CADR_REQUIRE(numJobs <= ws->maxJobs, "cadrCullBatch: more jobs than the workspace was created for");
-	const uint32_t tileSize = ke->block * ke->items;
 	uint64_t totalWork = 0, statusNeed = 0;
 	for (uint32_t j = 0; j < numJobs; j++) {
-		const cadrCullJob& J = hostJobs[j];
-		CADR_REQUIRE(J.numTiles == 0 || J.tiles, "cadrCullBatch: job without tiles");
-		CADR_REQUIRE(J.numTiles == 0 || (J.spheres && J.meshIndex && J.meshes && J.views && J.commands && J.counts),
-		             "cadrCullBatch: job with a null array");
-		CADR_REQUIRE(J.spheres % 16 == 0 && J.meshes % 16 == 0 && J.views % 16 == 0 && J.tiles % 16 == 0 &&
-		                 J.meshIndex % 4 == 0 && J.commands % 4 == 0 && J.counts % 4 == 0,
-		             "cadrCullBatch: misaligned job array (spheres, meshes, views, tiles need 16 bytes)");
-		CADR_REQUIRE(J.commandStride >= J.numDrawables, "cadrCullBatch: commandStride < numDrawables");
-		CADR_REQUIRE(J.numTiles == 0 || J.numViews > 0 || true, "");
-		totalWork += J.numTiles;
-		statusNeed += uint64_t(J.numViews) * J.numTiles;
+		const int rc = checkJob(hostJobs[j]);
+		if (rc != CADR_SUCCESS) return rc;
+		totalWork += hostJobs[j].numTiles;
+		statusNeed += uint64_t(hostJobs[j].numViews) * hostJobs[j].numTiles;
 	}
 	CADR_REQUIRE(totalWork < 0xffffffffull, "cadrCullBatch: too many tiles in one launch");
 	CADR_REQUIRE(statusNeed <= ws->statusWords, "cadrCullBatch: workspace has too few status words");
-	(void)tileSize;
 	int dev = 0;
 	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
 	CADR_REQUIRE(dev == ws->device, "cadrCullBatch: workspace belongs to another device");
 
 	hipStream_t s = hipStream_t(stream);
 	const auto* jobs = reinterpret_cast<const cadrCullJob*>(deviceJobs);
-	hipLaunchKernelGGL(cullSetupKernel, dim3(1), dim3(256), 0, s, jobs, numJobs, ws->firstWork, ws->statusBase, ws->ctl);
+	hipLaunchKernelGGL(cullSetupKernel, dim3(1), dim3(256), 0, s, jobs, numJobs, ws->firstWork, ws->statusBase,
+	                   ws->ctl);
 	CADR_CHECK_HIP(hipGetLastError(), "cullSetupKernel launch");
 	if (totalWork == 0) return ok();
 
@@ -624,14 +845,35 @@ CADR_API int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, co
 	p.numJobs = numJobs;
 	p.totalWork = uint32_t(totalWork);
 	p.flags = config->flags;
-	hipLaunchKernelGGL(ke->fn, dim3(uint32_t(grid)), dim3(ke->block), 0, s, p);
+	hipLaunchKernelGGL(ke->batch, dim3(uint32_t(grid)), dim3(ke->block), 0, s, p);
 	CADR_CHECK_HIP(hipGetLastError(), "cullKernel launch");
 	return ok();
 }
 
 CADR_API int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCullJob* hostJob,
-                          cadrCullWorkspace* workspace, cadrStream stream)
+                          cadrCullWorkspace* ws, cadrStream stream)
 {
 	CADR_REQUIRE(hostJob && hostJob->numViews == 1, "cadrCullView: the job must hold exactly one view");
-	return cadrCullBatch(config, deviceJob, hostJob, 1, workspace, stream);
+	const KernelEntry* ke = findKernel(config);
+	CADR_REQUIRE(ke, "cadrCullView: unsupported blockSize/itemsPerThread");
+	if (!ke->view) return cadrCullBatch(config, deviceJob, hostJob, 1, ws, stream);
+	CADR_REQUIRE(ws, "cadrCullView: workspace is NULL");
+	CADR_REQUIRE(deviceJob && deviceJob % 16 == 0, "cadrCullView: deviceJob must be a 16-byte aligned address");
+	const int rc = checkJob(*hostJob);
+	if (rc != CADR_SUCCESS) return rc;
+	CADR_REQUIRE(hostJob->numTiles <= ws->statusWords, "cadrCullView: workspace has too few status words");
+	int dev = 0;
+	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
+	CADR_REQUIRE(dev == ws->device, "cadrCullView: workspace belongs to another device");
+	if (hostJob->numTiles == 0) return ok();
+	CullParams p{};
+	p.jobs = reinterpret_cast<const cadrCullJob*>(deviceJob);
+	p.status = ws->status;
+	p.ctl = ws->ctl;
+	p.numJobs = 1;
+	p.totalWork = hostJob->numTiles;
+	p.flags = config->flags;
+	hipLaunchKernelGGL(ke->view, dim3(hostJob->numTiles), dim3(256), 0, hipStream_t(stream), p);
+	CADR_CHECK_HIP(hipGetLastError(), "cullViewKernel launch");
+	return ok();
 }

@@ -1,0 +1,217 @@
+// Geometry, MatrixList, Drawable and StateSet of the CadR facade.
+#include <algorithm>
+#include <cstring>
+
+#include "../../../include/CadR/Drawable.h"
+#include "../../../include/CadR/Exceptions.h"
+#include "../../../include/CadR/Renderer.h"
+
+namespace CadR {
+
+Geometry::Geometry(Renderer& r)
+    : _vertices(r.dataStorage()), _indices(r.dataStorage()), _primitiveSets(r.dataStorage())
+{
+}
+
+// ------------------------------------------------------------ MatrixList
+
+MatrixList::MatrixList(Renderer& r) : _matrixList(r.dataStorage()) {}
+
+// Header {numMatrices, capacity = numMatrices, 0 x 7} then the matrices
+// (MatrixList.h:54-59: initZeroField writes both counts and zeroes the rest).
+float* MatrixList::editNewContent(size_t numMatrices)
+{
+	auto* p = static_cast<uint8_t*>(_matrixList.alloc(64 + 64 * numMatrices));
+	uint32_t hdr[16] = {};
+	hdr[0] = uint32_t(numMatrices);
+	hdr[1] = uint32_t(numMatrices);
+	std::memcpy(p, hdr, 64);
+	return reinterpret_cast<float*>(p + 64);
+}
+
+void MatrixList::setMatrices(const float* matrices, size_t numMatrices)
+{
+	float* m = editNewContent(numMatrices);
+	if (numMatrices) std::memcpy(m, matrices, 64 * numMatrices);
+}
+
+// -------------------------------------------------------------- Drawable
+
+static DrawableGpuData gpuDataOf(Geometry& g, uint32_t off, MatrixList& ml, DataAllocation* dd)
+{
+	return DrawableGpuData(g.vertexDataAllocation().handle(), g.indexDataAllocation().handle(), ml.handle(),
+	                       dd ? dd->handle() : 0, g.primitiveSetDataAllocation().handle(), off);
+}
+
+Drawable::Drawable(Geometry& g, uint32_t off, MatrixList& ml, StateSet& ss) : _matrixList(&ml)
+{
+	ss.appendDrawableInternal(*this, gpuDataOf(g, off, ml, nullptr));
+}
+
+Drawable::Drawable(Geometry& g, uint32_t off, MatrixList& ml, DataAllocation& dd, StateSet& ss)
+    : _matrixList(&ml), _drawableData(&dd)
+{
+	ss.appendDrawableInternal(*this, gpuDataOf(g, off, ml, &dd));
+}
+
+Drawable::Drawable(Drawable&& o) noexcept
+    : _stateSet(o._stateSet), _matrixList(o._matrixList), _drawableData(o._drawableData),
+      _indexIntoStateSet(o._indexIntoStateSet)
+{
+	if (_indexIntoStateSet != ~0u) _stateSet->_drawablePtrList[_indexIntoStateSet] = this;
+	o._indexIntoStateSet = ~0u;
+}
+
+Drawable& Drawable::operator=(Drawable&& rhs) noexcept
+{
+	if (this == &rhs) return *this;
+	if (_indexIntoStateSet != ~0u) _stateSet->removeDrawableInternal(*this);
+	_stateSet = rhs._stateSet;
+	_matrixList = rhs._matrixList;
+	_drawableData = rhs._drawableData;
+	_indexIntoStateSet = rhs._indexIntoStateSet;
+	if (_indexIntoStateSet != ~0u) _stateSet->_drawablePtrList[_indexIntoStateSet] = this;
+	rhs._indexIntoStateSet = ~0u;
+	return *this;
+}
+
+Drawable::~Drawable() noexcept
+{
+	if (_indexIntoStateSet != ~0u) _stateSet->removeDrawableInternal(*this);
+}
+
+void Drawable::destroy() noexcept
+{
+	if (_indexIntoStateSet != ~0u) {
+		_stateSet->removeDrawableInternal(*this);
+		_indexIntoStateSet = ~0u;
+	}
+}
+
+// Drawable::create (Drawable.cpp:108-150): same StateSet -> only the record
+// is rewritten in place; another StateSet -> removed there, appended here.
+// As in the reference, the in-place update does not carry drawableData.
+void Drawable::createInternal(Geometry& g, uint32_t off, MatrixList& ml, DataAllocation* dd, StateSet& ss)
+{
+	_matrixList = &ml;
+	_drawableData = dd;
+	if (_indexIntoStateSet != ~0u) {
+		if (_stateSet == &ss) {
+			_stateSet->_drawableDataList[_indexIntoStateSet] = gpuDataOf(g, off, ml, nullptr);
+			return;
+		}
+		_stateSet->removeDrawableInternal(*this);
+	}
+	ss.appendDrawableInternal(*this, gpuDataOf(g, off, ml, nullptr));
+}
+
+void Drawable::create(Geometry& g, uint32_t off, MatrixList& ml, StateSet& ss) { createInternal(g, off, ml, nullptr, ss); }
+
+void Drawable::create(Geometry& g, uint32_t off, MatrixList& ml, DataAllocation& dd, StateSet& ss)
+{
+	createInternal(g, off, ml, &dd, ss);
+}
+
+// -------------------------------------------------------------- StateSet
+
+StateSet::~StateSet() noexcept
+{
+	removeAllDrawables();
+	for (StateSet* c : _children) {
+		auto& p = c->_parents;
+		p.erase(std::find(p.begin(), p.end(), this));
+	}
+	for (StateSet* p : _parents) {
+		auto& c = p->_children;
+		c.erase(std::remove(c.begin(), c.end(), this), c.end());
+	}
+}
+
+void StateSet::appendChild(StateSet& child)
+{
+	_children.push_back(&child);
+	child._parents.push_back(this);
+}
+
+void StateSet::removeChild(StateSet& child)
+{
+	auto it = std::find(_children.begin(), _children.end(), &child);
+	if (it == _children.end()) throw LogicError("CadR::StateSet::removeChild(): not a child");
+	_children.erase(it);
+	auto& p = child._parents;
+	p.erase(std::find(p.begin(), p.end(), this));
+}
+
+void StateSet::appendDrawableInternal(Drawable& d, const DrawableGpuData& gpuData)
+{
+	d._stateSet = this;
+	d._indexIntoStateSet = uint32_t(_drawableDataList.size());
+	_drawableDataList.emplace_back(gpuData);
+	_drawablePtrList.emplace_back(&d);
+}
+
+// Removal moves the last drawable into the hole (StateSet.cpp:29-47), so the
+// order inside a StateSet is vector order, not creation order.
+void StateSet::removeDrawableInternal(Drawable& d) noexcept
+{
+	const uint32_t i = d._indexIntoStateSet;
+	const size_t last = _drawableDataList.size() - 1;
+	if (i != last) {
+		_drawableDataList[i] = _drawableDataList[last];
+		Drawable* moved = _drawablePtrList[last];
+		_drawablePtrList[i] = moved;
+		moved->_indexIntoStateSet = i;
+	}
+	_drawableDataList.pop_back();
+	_drawablePtrList.pop_back();
+}
+
+void StateSet::appendDrawable(Drawable& d, const DrawableGpuData& gpuData)
+{
+	if (d._indexIntoStateSet != ~0u) d._stateSet->removeDrawableInternal(d);
+	appendDrawableInternal(d, gpuData);
+}
+
+void StateSet::removeDrawable(Drawable& d)
+{
+	if (d._indexIntoStateSet == ~0u) return;
+	d._stateSet->removeDrawableInternal(d);
+	d._indexIntoStateSet = ~0u;
+}
+
+void StateSet::removeAllDrawables() noexcept
+{
+	for (Drawable* d : _drawablePtrList) d->_indexIntoStateSet = ~0u;
+	_drawableDataList.clear();
+	_drawablePtrList.clear();
+}
+
+// StateSet::prepareRecording (StateSet.cpp:180-195): drawables of this
+// StateSet and its subtree, children visited once per parent edge.
+size_t StateSet::prepareRecording()
+{
+	_skipRecording = !_forceRecording;
+	size_t n = _drawableDataList.size();
+	for (StateSet* c : _children) {
+		n += c->prepareRecording();
+		_skipRecording = _skipRecording && c->_skipRecording;
+	}
+	_skipRecording = _skipRecording && n == 0;
+	return n;
+}
+
+// StateSet::recordToCommandBuffer (StateSet.cpp:198-268): this StateSet's
+// drawables go to [drawableCounter, drawableCounter + n), then the children.
+void StateSet::recordToCommandBuffer(DrawableGpuData* staging, std::vector<DrawRange>& ranges, size_t& counter)
+{
+	if (_skipRecording) return;
+	const size_t n = _drawableDataList.size();
+	if (n > 0) {
+		if (staging) std::memcpy(staging + counter, _drawableDataList.data(), n * sizeof(DrawableGpuData));
+		ranges.push_back(DrawRange{this, counter, n});
+		counter += n;
+	}
+	for (StateSet* c : _children) c->recordToCommandBuffer(staging, ranges, counter);
+}
+
+}  // namespace CadR

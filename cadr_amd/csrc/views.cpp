@@ -91,7 +91,7 @@ CADR_API int cadrMakeView(const float eye[3], const float dir[3], const float up
 static_assert(sizeof(cadrView) == 96, "cadrView is 96 bytes");
 static_assert(sizeof(cadrMesh) == 64, "cadrMesh is 64 bytes");
 static_assert(sizeof(cadrCullJob) == 80, "cadrCullJob is 80 bytes");
-static_assert(sizeof(cadrCullTile) == 16, "cadrCullTile is 16 bytes");
+static_assert(sizeof(cadrCullTile) == 32, "cadrCullTile is 32 bytes");
 static_assert(sizeof(cadrDrawIndexedIndirectCommand) == 20, "VkDrawIndexedIndirectCommand is 20 bytes");
 static_assert(sizeof(cadrTransformLevel) == 72, "cadrTransformLevel is 72 bytes");
 static_assert(sizeof(cadrDrawableGpuData) == 48, "DrawableGpuData is 48 bytes");

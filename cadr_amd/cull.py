@@ -201,6 +201,12 @@ class CullBatch:
         N.check(N.load().cadrCullBatch(ctypes.byref(self.config), self.device_jobs.address, N.ptr(self.host_jobs),
                                        self.host_jobs.shape[0], self.workspace.handle, stream))
 
+    def run_view(self, stream=None) -> None:
+        """The single-view pass (cadrCullView) for a batch of one job with one view."""
+        assert self.host_jobs.shape[0] == 1 and self.host_jobs["numViews"][0] == 1
+        N.check(N.load().cadrCullView(ctypes.byref(self.config), self.device_jobs.address, N.ptr(self.host_jobs),
+                                      self.workspace.handle, stream))
+
     def close(self):
         self.device_jobs.close()
         for b in self._keep:

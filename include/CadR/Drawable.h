@@ -1,0 +1,166 @@
+// Geometry, MatrixList, Drawable and StateSet of the CadR facade.
+//
+// Same roles and semantics as the reference (src/CadR/Geometry.h,
+// MatrixList.h, Drawable.h, StateSet.h):
+//   - Geometry owns vertex, index and PrimitiveSet allocations, one handle each;
+//   - MatrixList owns {numMatrices, capacity, pad[7]} + mat4[] (MatrixList.h:54-59);
+//   - a Drawable binds Geometry + PrimitiveSet byte offset + MatrixList
+//     (+ optional per-drawable data) into a 48-byte DrawableGpuData kept by
+//     its StateSet (Drawable.cpp:35-78);
+//   - a StateSet keeps its drawables in a vector; removal swaps the last
+//     element into the hole (StateSet.cpp:29-47); child StateSets are visited
+//     depth first, and a StateSet reachable through several parents is
+//     visited once per path (StateSet.cpp:180-268).
+#pragma once
+
+#include "Export.h"
+
+#include <cstdint>
+#include <vector>
+
+#include "DataStorage.h"
+
+namespace CadR {
+
+class Renderer;
+class StateSet;
+
+struct DrawableGpuData {
+	uint64_t vertexDataHandle;
+	uint64_t indexDataHandle;
+	uint64_t matrixListHandle;
+	uint64_t drawableDataHandle;
+	uint64_t primitiveSetHandle;
+	uint32_t primitiveSetOffset;
+	uint32_t padding;
+	DrawableGpuData() = default;
+	constexpr DrawableGpuData(uint64_t v, uint64_t i, uint64_t m, uint64_t d, uint64_t p, uint32_t off)
+	    : vertexDataHandle(v), indexDataHandle(i), matrixListHandle(m), drawableDataHandle(d),
+	      primitiveSetHandle(p), primitiveSetOffset(off), padding(0) {}
+};
+static_assert(sizeof(DrawableGpuData) == 48, "DrawableGpuData is 48 bytes (Drawable.cpp:13)");
+
+struct PrimitiveSet {
+	uint32_t indexCount;
+	uint32_t startIndex;
+};
+
+class CADR_EXPORT Geometry {
+public:
+	explicit Geometry(Renderer& r);
+	Geometry(const Geometry&) = delete;
+	Geometry& operator=(const Geometry&) = delete;
+
+	DataAllocation& vertexDataAllocation() { return _vertices; }
+	DataAllocation& indexDataAllocation() { return _indices; }
+	DataAllocation& primitiveSetDataAllocation() { return _primitiveSets; }
+	const DataAllocation& vertexDataAllocation() const { return _vertices; }
+	const DataAllocation& indexDataAllocation() const { return _indices; }
+	const DataAllocation& primitiveSetDataAllocation() const { return _primitiveSets; }
+
+	void uploadVertexData(const void* ptr, size_t numBytes) { _vertices.upload(ptr, numBytes); }
+	void uploadIndexData(const void* ptr, size_t numBytes) { _indices.upload(ptr, numBytes); }
+	void uploadPrimitiveSetData(const void* ptr, size_t numBytes) { _primitiveSets.upload(ptr, numBytes); }
+	void* createVertexStagingData(size_t numBytes) { return _vertices.alloc(numBytes); }
+	void* createIndexStagingData(size_t numBytes) { return _indices.alloc(numBytes); }
+	void* createPrimitiveSetStagingData(size_t numBytes) { return _primitiveSets.alloc(numBytes); }
+	void freeData() { _vertices.free(); _indices.free(); _primitiveSets.free(); }
+
+private:
+	DataAllocation _vertices, _indices, _primitiveSets;
+};
+
+class CADR_EXPORT MatrixList {
+public:
+	explicit MatrixList(Renderer& r);
+	// column-major float mat4 values, 16 floats each
+	void setMatrices(const float* matrices, size_t numMatrices);
+	float* editNewContent(size_t numMatrices);
+	uint64_t handle() const { return _matrixList.handle(); }
+	uint64_t deviceAddress() const { return _matrixList.deviceAddress(); }
+
+private:
+	DataAllocation _matrixList;
+};
+
+class CADR_EXPORT Drawable {
+public:
+	Drawable() noexcept = default;
+	Drawable(Geometry& geometry, uint32_t primitiveSetOffset, MatrixList& matrixList, StateSet& stateSet);
+	Drawable(Geometry& geometry, uint32_t primitiveSetOffset, MatrixList& matrixList, DataAllocation& drawableData,
+	         StateSet& stateSet);
+	Drawable(const Drawable&) = delete;
+	Drawable& operator=(const Drawable&) = delete;
+	Drawable(Drawable&& other) noexcept;
+	Drawable& operator=(Drawable&& rhs) noexcept;
+	~Drawable() noexcept;
+
+	void create(Geometry& geometry, uint32_t primitiveSetOffset, MatrixList& matrixList, StateSet& stateSet);
+	void create(Geometry& geometry, uint32_t primitiveSetOffset, MatrixList& matrixList, DataAllocation& drawableData,
+	            StateSet& stateSet);
+	void destroy() noexcept;
+	bool isValid() const { return _indexIntoStateSet != ~0u; }
+	StateSet& stateSet() const { return *_stateSet; }
+	uint32_t indexIntoStateSet() const { return _indexIntoStateSet; }
+
+private:
+	void createInternal(Geometry& g, uint32_t off, MatrixList& ml, DataAllocation* dd, StateSet& ss);
+	StateSet* _stateSet = nullptr;
+	MatrixList* _matrixList = nullptr;
+	DataAllocation* _drawableData = nullptr;
+	uint32_t _indexIntoStateSet = ~0u;
+	friend class StateSet;
+};
+
+// One StateSet's share of the indirect buffers, as StateSet::recordToCommandBuffer
+// issues it: commands [base, base + count) (StateSet.cpp:233-265).
+struct DrawRange {
+	StateSet* stateSet;
+	size_t base;
+	size_t count;
+};
+
+class CADR_EXPORT StateSet {
+public:
+	explicit StateSet(Renderer& r) noexcept : _renderer(&r) {}
+	~StateSet() noexcept;
+	StateSet(const StateSet&) = delete;
+	StateSet& operator=(const StateSet&) = delete;
+
+	Renderer& renderer() const { return *_renderer; }
+	// parent-child relation (ParentChildList.h): a child may have several parents
+	void appendChild(StateSet& child);
+	void removeChild(StateSet& child);
+	const std::vector<StateSet*>& childList() const { return _children; }
+	const std::vector<StateSet*>& parentList() const { return _parents; }
+
+	// rendering (StateSet.cpp:180-268)
+	size_t prepareRecording();
+	void setForceRecording(bool v) { _forceRecording = v; }
+	bool skipRecording() const { return _skipRecording; }
+	// appends this subtree's drawables (DFS) to the staging list and its draw ranges
+	void recordToCommandBuffer(DrawableGpuData* staging, std::vector<DrawRange>& ranges, size_t& drawableCounter);
+
+	// drawables
+	void appendDrawable(Drawable& d, const DrawableGpuData& gpuData);
+	static void removeDrawable(Drawable& d);
+	void removeAllDrawables() noexcept;
+	Drawable& getDrawable(size_t index) const { return *_drawablePtrList[index]; }
+	size_t getNumDrawables() const { return _drawablePtrList.size(); }
+	const std::vector<DrawableGpuData>& drawableDataList() const { return _drawableDataList; }
+
+private:
+	void appendDrawableInternal(Drawable& d, const DrawableGpuData& gpuData);
+	void removeDrawableInternal(Drawable& d) noexcept;
+
+	Renderer* _renderer;
+	bool _skipRecording = true;
+	bool _forceRecording = false;
+	std::vector<DrawableGpuData> _drawableDataList;
+	std::vector<Drawable*> _drawablePtrList;
+	std::vector<StateSet*> _children;
+	std::vector<StateSet*> _parents;
+	friend class Drawable;
+};
+
+}  // namespace CadR

@@ -1,0 +1,122 @@
+// Renderer of the CadR facade: the frame API around the drawable-processing
+// pass, on one HIP device and caller-owned HIP streams.
+//
+// Mirrors CadR::Renderer (src/CadR/Renderer.h:34-201) for this path:
+//   beginFrame -> executeCopyOperations -> beginRecording ->
+//   prepareSceneRendering(root) -> recordDrawableProcessing(stream, n) ->
+//   recordSceneRendering(stream, root, ranges) -> endRecording -> endFrame.
+// prepareSceneRendering counts the drawables (StateSet::prepareRecording)
+// and grows the four per-frame buffers to ceil(1.2 N), at least 128 slots
+// (Renderer.cpp:446-595): drawable list 48 B, host staging 48 B, indirect
+// commands 16 B, DrawablePointers 32 B.  recordDrawableProcessing fills the
+// staging list in StateSet DFS order, copies it to the device and launches
+// the gather kernel with the push constants of processDrawables.comp:68-74
+// for the current handle-table depth (Renderer.cpp:598-720).  A HIP stream
+// starts executing work when it is enqueued (a Vulkan command buffer only at
+// submission), so the staging fill that the reference performs while
+// recording the draws (StateSet.cpp:233-237) happens before the copy here;
+// recordSceneRendering then only reports the per-StateSet draw ranges.
+//
+// FrameInfo carries the reference's fields (FrameInfo.h:15-32); GPU
+// timestamps are HIP events converted to nanoseconds from the frame's first
+// GPU event.
+#pragma once
+
+#include "Export.h"
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "DataStorage.h"
+#include "Drawable.h"
+
+namespace CadR {
+
+struct FrameInfo {
+	size_t frameNumber = size_t(-1);
+	uint64_t gpuBeginFrame = 0;
+	uint64_t cpuBeginFrame = 0;
+	uint64_t cpuPrepareRecordingBegin = 0;
+	uint64_t cpuPrepareRecordingEnd = 0;
+	uint64_t cpuRecordStateSetsBegin = 0;
+	uint64_t cpuRecordStateSetsEnd = 0;
+	std::vector<uint64_t> cpuUserTimestampList;
+	uint64_t cpuEndFrame = 0;
+	uint64_t gpuBeginExecution = 0;
+	uint64_t gpuAfterTransfersAndBeforeDrawableProcessing = 0;
+	uint64_t gpuAfterDrawableProcessingAndBeforeRendering = 0;
+	uint64_t gpuEndExecution = 0;
+	static constexpr uint32_t gpuTimestampPoolSize = 4;
+};
+
+class CADR_EXPORT Renderer {
+public:
+	static constexpr size_t smallMemorySize = DataStorage::smallMemorySize;
+	static constexpr size_t mediumMemorySize = DataStorage::mediumMemorySize;
+	static constexpr size_t largeMemorySize = DataStorage::largeMemorySize;
+	static constexpr uint32_t drawablePointersRecordSize = 4 * sizeof(uint64_t);
+
+	// device-less renderer: host memory, bookkeeping only (Renderer.cpp:64-74)
+	explicit Renderer(bool makeDefault = true);
+	// renderer on HIP device deviceIndex
+	explicit Renderer(int deviceIndex, bool makeDefault = true);
+	~Renderer();
+	Renderer(const Renderer&) = delete;
+	Renderer& operator=(const Renderer&) = delete;
+	void finalize();
+
+	static Renderer& get() { return *_defaultRenderer; }
+	static void set(Renderer& r) noexcept { _defaultRenderer = &r; }
+
+	// frame API
+	size_t beginFrame();
+	void executeCopyOperations();
+	void beginRecording(void* stream);
+	size_t prepareSceneRendering(StateSet& root);
+	void recordDrawableProcessing(void* stream, size_t numDrawables);
+	void recordSceneRendering(void* stream, StateSet& root, std::vector<DrawRange>& ranges);
+	void endRecording(void* stream);
+	void endFrame();
+
+	// getters
+	bool hasDevice() const { return _device >= 0; }
+	int deviceIndex() const { return _device; }
+	size_t frameNumber() const noexcept { return _frameNumber; }
+	DataStorage& dataStorage() const { return *_dataStorage; }
+	uint64_t drawableBufferAddress() const { return _drawableBuffer; }
+	size_t drawableBufferSize() const { return _capacity * sizeof(DrawableGpuData); }
+	DrawableGpuData* drawableStagingData() const { return _staging; }
+	uint64_t drawIndirectBufferAddress() const { return _indirectBuffer; }
+	uint64_t drawablePointersBufferAddress() const { return _pointersBuffer; }
+	size_t bufferCapacity() const { return _capacity; }
+
+	// frame timing (Renderer.cpp:1025-1066)
+	void setCollectFrameInfo(bool on) { _collectFrameInfo = on; }
+	bool collectFrameInfo() const { return _collectFrameInfo; }
+	const FrameInfo& getFrameInfo();
+	static uint64_t getCpuTimestamp();
+	static double cpuTimestampPeriod() { return 1e-9; }
+	static double gpuTimestampPeriod() { return 1e-9; }
+
+private:
+	void releaseBuffers() noexcept;
+	void recordEvent(int index, void* stream);
+
+	static Renderer* _defaultRenderer;
+	int _device = -1;
+	std::unique_ptr<DataStorage> _dataStorage;
+	size_t _frameNumber = size_t(-1);
+	size_t _capacity = 0;
+	uint64_t _drawableBuffer = 0, _indirectBuffer = 0, _pointersBuffer = 0;
+	DrawableGpuData* _staging = nullptr;
+	std::vector<DrawRange> _pendingRanges;
+	StateSet* _preparedRoot = nullptr;
+	bool _collectFrameInfo = false;
+	FrameInfo _inProgress, _completed;
+	void* _events[FrameInfo::gpuTimestampPoolSize] = {};
+	bool _eventRecorded[FrameInfo::gpuTimestampPoolSize] = {};
+	size_t _infoFrame = size_t(-1);
+};
+
+}  // namespace CadR

@@ -271,19 +271,29 @@ int cadrMakeView(const float eye[3], const float dir[3], const float up[3], floa
                  float aspect, float zNear, float zFar, float viewportHeight,
                  const float* lodThresholds, cadrView* out);
 
-/* A cull tile: a run of at most tileSize drawables inside one StateSet. */
+/* A cull tile: a run of at most tileSize drawables inside one StateSet.
+ * Tiles are processed in array order (the ticket order); chainIndex numbers
+ * the tiles StateSet by StateSet (all tiles of StateSet 0, then of StateSet 1,
+ * ...) and indexes the look-back status words, so the predecessor of a tile in
+ * its StateSet is the tile with chainIndex - 1, which must come earlier in the
+ * array. */
 typedef struct cadrCullTile {
 	uint32_t start;             /* first drawable                                  */
 	uint32_t count;             /* 0 only for the single tile of an empty StateSet */
 	uint32_t stateSetBase;      /* base_k of the tile's StateSet                   */
 	uint32_t stateSetAndFlags;  /* k | CADR_TILE_FIRST | CADR_TILE_LAST            */
-} cadrCullTile;
+	uint32_t chainIndex;        /* StateSet-major position of the tile             */
+	uint32_t reserved[3];
+} cadrCullTile;                 /* 32 B */
 #define CADR_TILE_FIRST 0x80000000u
 #define CADR_TILE_LAST  0x40000000u
 #define CADR_TILE_STATESET_MASK 0x3fffffffu
 
 /* Splits StateSets of sizes stateSetCounts[0..numStateSets) into tiles of at
- * most tileSize drawables.  With tiles == NULL it only counts them. */
+ * most tileSize drawables, emitted round-robin over the StateSets (the q-th
+ * tile of every StateSet, then the (q+1)-th, ...): consecutive tiles of one
+ * StateSet are numStateSets tickets apart, which keeps look-back walks short.
+ * With tiles == NULL it only counts them. */
 int cadrBuildCullTiles(const uint32_t* stateSetCounts, uint32_t numStateSets, uint32_t tileSize,
                        cadrCullTile* tiles, uint32_t capacity, uint32_t* numTiles);
 

@@ -72,12 +72,21 @@ def test_build_tiles_layout():
         assert tile["count"] <= 1024
         covered[tile["start"]: tile["start"] + tile["count"]] += 1
     assert np.all(covered == 1)
+    chain_base = 0
     for k in range(len(counts)):
         ts = per_k[k]
         assert int(ts[0]["stateSetAndFlags"]) & N.TILE_FIRST and int(ts[-1]["stateSetAndFlags"]) & N.TILE_LAST
         assert len(ts) == max(1, -(-int(counts[k]) // 1024))
         if counts[k] == 0:
             assert ts[0]["count"] == 0
+        # chain indices number the tiles StateSet by StateSet, in drawable order
+        assert [int(x["chainIndex"]) for x in ts] == list(range(chain_base, chain_base + len(ts)))
+        assert all(ts[i]["start"] < ts[i + 1]["start"] for i in range(len(ts) - 1))
+        chain_base += len(ts)
+    # round-robin issue order: the q-th tile of every StateSet before any (q+1)-th
+    q_of = [(int(x["start"]) - int(x["stateSetBase"])) // 1024 for x in t]
+    assert q_of == sorted(q_of)
+    assert sorted(int(x["chainIndex"]) for x in t) == list(range(len(t)))
     # counting mode agrees; a short buffer is an error
     n = ctypes.c_uint32(0)
     assert N.load().cadrBuildCullTiles(N.ptr(counts), counts.size, 1024, None, 0, ctypes.byref(n)) == 0

@@ -22,10 +22,13 @@ def _compare(cmds, counts, exp_cmds, exp_counts, stateset_counts, what=""):
                 raise AssertionError(f"{what}: view {v} StateSet {k} commands differ at {bad}: {a[bad]} vs {b[bad]}")
 
 
-def _run(ds, views, config, stride=None):
+def _run(ds, views, config, stride=None, single=False):
     out = cull.CullOutputs(views.shape[0], stride or ds.n, ds.K)
     b = cull.CullBatch(config, [(ds, views, out)])
-    b.run()
+    if single:
+        b.run_view()
+    else:
+        b.run()
     from cadr_amd.device import synchronize
     synchronize()
     cmds, counts = out.download()
@@ -80,13 +83,15 @@ def plain(gpu):
     ds.close()
 
 
-@pytest.mark.parametrize("block,items", [(256, 2), (256, 4), (256, 8), (512, 2), (512, 4), (128, 8), (128, 16)])
+@pytest.mark.parametrize("block,items", [(256, 2), (256, 4), (256, 8), (256, 16), (512, 2), (512, 4), (128, 8),
+                                         (128, 16)])
 @pytest.mark.parametrize("flags", [0, N.CULL_NO_TILE_SPHERES])
-def test_cull_plain100k_single_view(plain, block, items, flags):
+@pytest.mark.parametrize("path", ["view", "batch"])
+def test_cull_plain100k_single_view(plain, block, items, flags, path):
     scene, ds, spheres = plain
-    print(f"plain100k block={block} items={items} flags={flags}", flush=True)
+    print(f"plain100k {path} block={block} items={items} flags={flags}", flush=True)
     views = cull.make_views([scenes.DEFAULT_CAMERA])
-    cmds, counts = _run(ds, views, cull.make_config(block, items, flags=flags))
+    cmds, counts = _run(ds, views, cull.make_config(block, items, flags=flags), single=(path == "view"))
     exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
                                        cmd_dtype=N.CMD_DTYPE)
     assert exp_counts.sum() > 1000
@@ -190,3 +195,37 @@ def test_cull_edge_cases(gpu):
                      counts_in, name)
             b.close(); out.close()
         ds.close()
+
+
+def test_cull_view_after_batch_shares_workspace(plain):
+    """Single-view launches, a batched launch and single-view launches again on one
+    workspace: each launch leaves the ticket reset and the epoch advanced."""
+    from cadr_amd.device import synchronize
+    scene, ds, spheres = plain
+    cams = scenes.batch_cameras(5)
+    views = cull.make_views(cams)
+    exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
+                                       cmd_dtype=N.CMD_DTYPE)
+    cfg = cull.make_config(256, 4)
+    ws = cull.Workspace(5 * 200, 8)
+    outs = [cull.CullOutputs(1, ds.n, ds.K) for _ in range(5)]
+    singles = [cull.CullBatch(cfg, [(ds, views[i:i + 1], outs[i])], workspace=ws) for i in range(5)]
+    multi_out = cull.CullOutputs(5, ds.n, ds.K)
+    multi = cull.CullBatch(cfg, [(ds, views, multi_out)], workspace=ws)
+    for rep in range(3):
+        for b in singles[:2]:
+            b.run_view()
+        multi.run()
+        for b in singles[2:]:
+            b.run_view()
+        synchronize()
+        for i in range(5):
+            c, k = outs[i].download()
+            _compare(c, k, exp_cmds[i:i + 1], exp_counts[i:i + 1], scene.stateset_counts, f"single {i} rep {rep}")
+        c, k = multi_out.download()
+        _compare(c, k, exp_cmds, exp_counts, scene.stateset_counts, f"multi rep {rep}")
+    for b in singles + [multi]:
+        b.close()
+    for o in outs + [multi_out]:
+        o.close()
+    ws.close()
