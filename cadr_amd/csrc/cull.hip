@@ -61,10 +61,46 @@ struct CullParams {
 	uint32_t flags;
 };
 
+// Pointers read from job descriptors are generic; loads through the global
+// address space become global_load (not flat) and leave the LDS counter alone.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> globalPtr(uint64_t a)
+{
+	return reinterpret_cast<gptr<T>>(a);
+}
+
 struct TileDesc {
 	uint32_t start, count, segBase, k, chain;
 	bool first, last;
 };
+
+// Loads item j = base + 64*j of a tile for every j, all issued before any is
+// used: indices are clamped into the tile (a partial tile re-reads its last
+// drawable), so no load sits behind a per-lane branch; an empty tile loads
+// nothing.  Callers mask items at or beyond T.count.
+template <int ITEMS>
+__device__ __forceinline__ void loadTileItems(gptr<float4> spheres, gptr<uint32_t> meshIndex, const TileDesc& T,
+                                              uint32_t base, float4 (&sp)[ITEMS], uint32_t (&mi)[ITEMS])
+{
+	if (T.count == 0) {
+#pragma unroll
+		for (int j = 0; j < ITEMS; j++) {
+			sp[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+			mi[j] = 0;
+		}
+		return;
+	}
+	const uint32_t last = T.count - 1;
+#pragma unroll
+	for (int j = 0; j < ITEMS; j++) {
+		const uint32_t e = min(base + uint32_t(j) * 64u, last);
+		const gptr<float4> p = spheres + (T.start + e);
+		sp[j] = make_float4(p->x, p->y, p->z, p->w);
+		mi[j] = meshIndex[T.start + e];
+	}
+}
 
 __device__ __forceinline__ TileDesc loadTile(const cadrCullJob& J, uint32_t t, uint32_t tileSize)
 {
@@ -257,13 +293,32 @@ __device__ uint32_t lookBackWave(const uint64_t* chain, uint32_t c, uint32_t epo
 	return excl;
 }
 
-__device__ __forceinline__ void writeCommand(uint32_t* o, const cadrMesh& mesh, uint32_t lod, uint32_t drawable)
+// A wave's compacted commands are staged in LDS (5 dwords each, a stride
+// that is conflict-free across the LDS banks) and then stored as runs of 64
+// consecutive dwords, so every store instruction writes 256 contiguous bytes
+// and HBM sees whole lines instead of partial ones.
+__device__ __forceinline__ void stageCommand(uint32_t* stage, uint32_t slot, const cadrMesh& mesh, uint32_t lod,
+                                             uint32_t drawable)
 {
-	__builtin_nontemporal_store(mesh.indexCount[lod], o + 0);
-	__builtin_nontemporal_store(1u, o + 1);
-	__builtin_nontemporal_store(mesh.firstIndex[lod], o + 2);
-	__builtin_nontemporal_store(uint32_t(mesh.vertexOffset), o + 3);
-	__builtin_nontemporal_store(drawable, o + 4);
+	uint32_t* o = stage + slot * 5u;
+	o[0] = mesh.indexCount[lod];
+	o[1] = 1u;
+	o[2] = mesh.firstIndex[lod];
+	o[3] = uint32_t(mesh.vertexOffset);
+	o[4] = drawable;
+}
+
+// Stores n staged commands (n <= 64) to commands[first .. first+n), skipping
+// slots at or beyond the stride.
+__device__ __forceinline__ void flushCommands(uint32_t* out, const uint32_t* stage, uint32_t first, uint32_t n,
+                                              uint32_t stride, uint32_t lane)
+{
+	uint32_t* dst = out + uint64_t(first) * 5ull;
+#pragma unroll
+	for (uint32_t k = 0; k < 5; k++) {
+		const uint32_t d = k * 64u + lane;
+		if (d < 5u * n && first + d / 5u < stride) __builtin_nontemporal_store(stage[d], dst + d);
+	}
 }
 
 // Mesh table of a job staged in LDS when it is small (one 16-byte word per
@@ -283,32 +338,26 @@ __device__ __forceinline__ bool stageMeshes(cadrMesh* sMeshes, const cadrCullJob
 template <int ITEMS>
 __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCullJob& J, uint32_t t, uint32_t epoch,
                                              const cadrMesh* meshes, uint32_t (*sCnt)[ITEMS], uint32_t* sExclP,
-                                             uint32_t tid, uint32_t lane, uint32_t wave)
+                                             uint32_t* sStage, uint32_t tid, uint32_t lane, uint32_t wave)
 {
 	constexpr int BLOCK = 256, WAVES = 4, WAVE_ITEMS = 64 * ITEMS;
 	const TileDesc T = loadTile(J, t, BLOCK * ITEMS);
 	const ViewC V = loadView(reinterpret_cast<const cadrView*>(J.views), 0);
-	const float4* __restrict__ spheres = reinterpret_cast<const float4*>(J.spheres);
-	const uint32_t* __restrict__ meshIndex = reinterpret_cast<const uint32_t*>(J.meshIndex);
+	const gptr<float4> spheres = globalPtr<float4>(J.spheres);
+	const gptr<uint32_t> meshIndex = globalPtr<uint32_t>(J.meshIndex);
 
-	float vz[ITEMS], rad[ITEMS];
+	// all loads first (indices clamped into the tile, so none is conditional),
+	// then the tests: a wave keeps ITEMS 16-byte loads in flight at once
+	float4 sp[ITEMS];
 	uint32_t mi[ITEMS];
+	loadTileItems<ITEMS>(spheres, meshIndex, T, wave * WAVE_ITEMS + lane, sp, mi);
+	float vz[ITEMS];
 	uint64_t mask[ITEMS];
 #pragma unroll
 	for (int j = 0; j < ITEMS; j++) {
 		const uint32_t e = wave * WAVE_ITEMS + j * 64 + lane;
-		bool vis = false;
-		vz[j] = 0.f;
-		rad[j] = 0.f;
-		mi[j] = 0;
-		if (e < T.count) {
-			const float4 s = spheres[T.start + e];
-			const uint32_t m = meshIndex[T.start + e];
-			vis = m < J.numMeshes && sphereVisible(s, V);
-			vz[j] = viewCoord(V.r2, s);
-			rad[j] = s.w;
-			mi[j] = m;
-		}
+		const bool vis = e < T.count && mi[j] < J.numMeshes && sphereVisible(sp[j], V);
+		vz[j] = viewCoord(V.r2, sp[j]);
 		mask[j] = __ballot(vis);
 		if (lane == 0) sCnt[wave][j] = uint32_t(__builtin_popcountll(mask[j]));
 	}
@@ -337,18 +386,22 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 	uint32_t pos = T.segBase + *sExclP;
 	for (uint32_t q = 0; q < wave; q++)
 		for (int j = 0; j < ITEMS; j++) pos += sCnt[q][j];
+	pos = uniform(pos);
 	uint32_t* out = reinterpret_cast<uint32_t*>(J.commands);
+	uint32_t* stage = sStage + wave * 320u;
 #pragma unroll
 	for (int j = 0; j < ITEMS; j++) {
+		const uint32_t n = uint32_t(__builtin_popcountll(mask[j]));
+		if (n == 0) continue;
 		if ((mask[j] >> lane) & 1ull) {
-			const uint32_t at = pos + maskRank(mask[j]);
-			if (at < J.commandStride) {
-				const cadrMesh& mesh = meshes[mi[j]];
-				const uint32_t lod = lodFromDepth(rad[j], vz[j], V, mesh.lodCount);
-				writeCommand(out + uint64_t(at) * 5ull, mesh, lod, T.start + wave * WAVE_ITEMS + j * 64 + lane);
-			}
+			const cadrMesh& mesh = meshes[mi[j]];
+			const uint32_t lod = lodFromDepth(sp[j].w, vz[j], V, mesh.lodCount);
+			stageCommand(stage, maskRank(mask[j]), mesh, lod, T.start + wave * WAVE_ITEMS + j * 64 + lane);
 		}
-		pos += uint32_t(__builtin_popcountll(mask[j]));
+		waveSync();
+		flushCommands(out, stage, pos, n, J.commandStride, lane);
+		waveSync();
+		pos += n;
 	}
 }
 
@@ -361,6 +414,7 @@ __global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
 	__shared__ uint32_t sCnt[WAVES][ITEMS];
 	__shared__ uint32_t sExcl;
 	__shared__ cadrMesh sMeshes[kLdsMeshes];
+	__shared__ uint32_t sStage[WAVES * 320];
 
 	const uint32_t tid = threadIdx.x, lane = laneId(), wave = tid / 64;
 	const uint32_t epoch = __hip_atomic_load(&P.ctl[kCtlEpoch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -370,7 +424,7 @@ __global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
 	__syncthreads();
 	const uint32_t t = uniform(sTicket);
 	if (t < P.totalWork) cullViewTile<ITEMS>(P, J, t, epoch, ldsMeshes ? sMeshes : reinterpret_cast<const cadrMesh*>(J.meshes),
-	                                         sCnt, &sExcl, tid, lane, wave);
+	                                         sCnt, &sExcl, sStage, tid, lane, wave);
 
 	// the last workgroup to finish resets the ticket and starts a new epoch
 	__syncthreads();
@@ -397,6 +451,7 @@ struct CullShared {
 	uint32_t cnt[kViewGroup][WAVES];
 	uint32_t excl[kViewGroup];
 	uint16_t list[WAVES][64 * ITEMS];
+	uint32_t stage[WAVES][320];
 	cadrMesh meshes[kLdsMeshes];
 	float red[WAVES][6];
 	uint32_t bad[WAVES];
@@ -438,24 +493,25 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		const bool ldsMeshes = stageMeshes<BLOCK>(sh.meshes, J, tid);
 
 		// load the tile: wave-contiguous runs, staged in LDS, bounds for the tile sphere
-		const float4* __restrict__ spheres = reinterpret_cast<const float4*>(J.spheres);
-		const uint32_t* __restrict__ meshIndex = reinterpret_cast<const uint32_t*>(J.meshIndex);
+		const gptr<float4> spheres = globalPtr<float4>(J.spheres);
+		const gptr<uint32_t> meshIndex = globalPtr<uint32_t>(J.meshIndex);
 		float4 sp[ITEMS];
+		uint32_t mi[ITEMS];
+		loadTileItems<ITEMS>(spheres, meshIndex, T, wave * WAVE_ITEMS + lane, sp, mi);
 		bool irregular = false;
 		float mn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
 		float mx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 #pragma unroll
 		for (int j = 0; j < ITEMS; j++) {
 			const uint32_t e = wave * WAVE_ITEMS + j * 64 + lane;
-			float4 s = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
-			uint32_t m = 0;
-			if (e < T.count) {
-				s = spheres[T.start + e];
-				m = meshIndex[T.start + e];
-				if (m >= J.numMeshes) {
-					s.w = -__builtin_inff();  // invalid mesh: never visible
-					m = 0;
-				}
+			float4 s = sp[j];
+			uint32_t m = mi[j];
+			if (e >= T.count) {
+				s = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
+				m = 0;
+			} else if (m >= J.numMeshes) {
+				s.w = -__builtin_inff();  // invalid mesh: never visible
+				m = 0;
 			}
 			sp[j] = s;
 			sh.sph[e] = s;
@@ -583,16 +639,19 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 				waveSync();
 				const ViewC V = loadView(views, g0 + v);
 				uint32_t* out = reinterpret_cast<uint32_t*>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
-				for (uint32_t r = lane; r < run; r += 64) {
-					const uint32_t e = wave * WAVE_ITEMS + sh.list[wave][r];
-					const uint32_t pos = T.segBase + wavePos + r;
-					if (pos >= J.commandStride) continue;
-					const float4 s = sh.sph[e];
-					const cadrMesh& mesh = meshes[sh.mesh[e]];
-					const uint32_t lod = lodFromDepth(s.w, viewCoord(V.r2, s), V, mesh.lodCount);
-					writeCommand(out + uint64_t(pos) * 5ull, mesh, lod, T.start + e);
+				for (uint32_t r0 = 0; r0 < run; r0 += 64) {
+					const uint32_t n = min(64u, run - r0);
+					if (lane < n) {
+						const uint32_t e = wave * WAVE_ITEMS + sh.list[wave][r0 + lane];
+						const float4 s = sh.sph[e];
+						const cadrMesh& mesh = meshes[sh.mesh[e]];
+						const uint32_t lod = lodFromDepth(s.w, viewCoord(V.r2, s), V, mesh.lodCount);
+						stageCommand(sh.stage[wave], lane, mesh, lod, T.start + e);
+					}
+					waveSync();
+					flushCommands(out, sh.stage[wave], T.segBase + wavePos + r0, n, J.commandStride, lane);
+					waveSync();
 				}
-				waveSync();
 			}
 			__syncthreads();
 		}

@@ -12,9 +12,9 @@
 // Memory: per node 64 B local read + 64 B world write (+4 B parent index);
 // parents are few and shared by many children, so their reads hit L2.  The
 // matrices are stored as the reference stores them (column-major glm::mat4,
-// 64 B), which keeps the world array usable as MatrixList payload; lane i
-// reads its 64 B as four 16-byte loads, and the four loads of a wave cover
-// one contiguous 4 KiB span.
+// 64 B), which keeps the world array usable as MatrixList payload; a wave's
+// 64 matrices are staged through LDS so that every global load and store
+// instruction moves one contiguous KiB.
 #include "internal.h"
 #include "fp32.h"
 
@@ -31,6 +31,44 @@ __device__ __forceinline__ float4 nodeSphere(const float4 W[4], const cadrMesh* 
 	return sphereTransform(W, local);
 }
 
+// A wave's 64 matrices (4 KiB, contiguous in memory) move between HBM and
+// registers through LDS: four 16-byte loads per lane at lane stride 16 B (each
+// instruction one contiguous KiB), then each lane reads its own 64 bytes from
+// LDS; stores go the same way in reverse.  Lanes past the end of the level
+// neither load nor store.
+__device__ __forceinline__ void loadMatrices(const float4* __restrict__ src, uint64_t first, uint64_t count,
+                                             float4* stage, uint32_t lane, float4 M[4])
+{
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		const uint32_t q = uint32_t(k) * 64u + lane;      // float4 index inside the wave's 4 KiB
+		if (first + q / 4u < count) stage[q] = src[4 * first + q];
+	}
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+	for (int c = 0; c < 4; c++) M[c] = stage[4 * lane + c];
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void storeMatrices(float4* __restrict__ dst, uint64_t first, uint64_t count,
+                                              float4* stage, uint32_t lane, const float4 M[4])
+{
+#pragma unroll
+	for (int c = 0; c < 4; c++) stage[4 * lane + c] = M[c];
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		const uint32_t q = uint32_t(k) * 64u + lane;
+		if (first + q / 4u < count) dst[4 * first + q] = stage[q];
+	}
+}
+
 template <bool ROOT, bool SPHERES>
 __global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict__ local,
                                                        const uint32_t* __restrict__ parentIndex,
@@ -41,17 +79,20 @@ __global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict_
                                                        const cadrMesh* __restrict__ meshes, uint32_t numMeshes,
                                                        float4* __restrict__ spheres)
 {
-	const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
-	if (i >= count) return;
+	__shared__ float4 stage[4][256];
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	const uint64_t first = uint64_t(blockIdx.x) * 256u + wave * 64u;  // the wave's first node
+	if (first >= count) return;  // whole wave past the end (waves are independent)
+	const uint64_t i = first + lane;
+	const bool live = i < count;
 	float4 L[4];
-#pragma unroll
-	for (int c = 0; c < 4; c++) L[c] = local[4 * i + c];
+	loadMatrices(local, first, count, stage[wave], lane, L);
 	float4 W[4];
 	if constexpr (ROOT) {
 #pragma unroll
 		for (int c = 0; c < 4; c++) W[c] = L[c];
 	} else {
-		const uint32_t p = parentIndex[i];
+		const uint32_t p = live ? parentIndex[i] : ~0u;
 		if (p < parentCount) {
 			float4 P[4];
 #pragma unroll
@@ -62,9 +103,10 @@ __global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict_
 			for (int c = 0; c < 4; c++) W[c] = L[c];
 		}
 	}
-#pragma unroll
-	for (int c = 0; c < 4; c++) world[4 * i + c] = W[c];
-	if constexpr (SPHERES) spheres[i] = nodeSphere(W, meshes, numMeshes, meshIndex[i]);
+	storeMatrices(world, first, count, stage[wave], lane, W);
+	if constexpr (SPHERES) {
+		if (live) spheres[i] = nodeSphere(W, meshes, numMeshes, meshIndex[i]);
+	}
 }
 
 __global__ __launch_bounds__(256) void worldSpheresKernel(const float4* __restrict__ world,

@@ -346,6 +346,93 @@ int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCul
 /* Kernel tile size for a config (blockSize * itemsPerThread), 0 if unsupported. */
 uint32_t cadrCullTileSize(const cadrCullConfig* config);
 
+/* ------------------------------------------------------------------------ */
+/* Serving: scenes resident in HBM, cull requests (scene, view) admitted by
+ * output memory and batch slots, and batched continuously — every launch
+ * takes the requests that are waiting when a slot frees, groups them by
+ * scene into jobs of one persistent cull launch, and several launches run
+ * at once on separate streams.  A request's result is exactly what
+ * cadrCullView computes for its view alone (same kernel arithmetic, same
+ * compaction order), whatever it was batched with.
+ * The reference has no serving mode; this is an extension of the path. */
+typedef struct cadrServer cadrServer;
+
+typedef struct cadrServerConfig {
+	cadrCullConfig cull;            /* kernel shape of every launch               */
+	uint32_t maxBatchViews;         /* views per launch (e.g. 256)                */
+	uint32_t maxInflightBatches;    /* launches in flight, one stream each        */
+	uint64_t outputBudgetBytes;     /* device memory for request outputs          */
+	uint32_t maxQueuedRequests;     /* submissions beyond this are refused        */
+	uint32_t reserved;
+} cadrServerConfig;
+
+typedef struct cadrServerScene {
+	uint64_t spheres;               /* float4[numDrawables], device, owned by the caller */
+	uint64_t meshIndex;             /* uint32[numDrawables], device              */
+	uint64_t meshes;                /* cadrMesh[numMeshes], device               */
+	const uint32_t* stateSetCounts; /* host, numStateSets entries                */
+	uint32_t numStateSets;
+	uint32_t numMeshes;
+	uint32_t numDrawables;
+	uint32_t reserved;
+} cadrServerScene;
+
+typedef struct cadrServerResult {
+	uint64_t requestId;
+	uint32_t sceneId;
+	uint32_t numStateSets;
+	uint64_t commands;              /* device cadrDrawIndexedIndirectCommand[numDrawables], StateSet ranges */
+	uint64_t counts;                /* device uint32[numStateSets]               */
+	uint64_t visible;               /* sum of the counts                         */
+	uint64_t submitNs;              /* steady-clock times of submission and of   */
+	uint64_t completeNs;            /* the counts being visible on the host      */
+} cadrServerResult;
+
+int cadrServerCreate(const cadrServerConfig* config, cadrServer** server);
+int cadrServerDestroy(cadrServer* server);
+/* Scenes are added before cadrServerStart; their arrays stay owned by the caller. */
+int cadrServerAddScene(cadrServer* server, const cadrServerScene* scene, uint32_t* sceneId);
+int cadrServerStart(cadrServer* server);
+/* Thread-safe.  CADR_ERROR_OUT_OF_RESOURCES when the queue is full or the
+ * request's output can never fit the budget. */
+int cadrServerSubmit(cadrServer* server, uint32_t sceneId, const cadrView* view, uint64_t* requestId);
+/* Completed requests, oldest first; count may be 0. */
+int cadrServerPoll(cadrServer* server, cadrServerResult* results, uint32_t capacity, uint32_t* count);
+/* Host copy of a completed request's per-StateSet counts. */
+int cadrServerCopyCounts(cadrServer* server, uint64_t requestId, uint32_t* counts, uint32_t capacity);
+/* Frees a completed request's output memory (its device arrays become invalid). */
+int cadrServerRelease(cadrServer* server, uint64_t requestId);
+/* Waits until no request is queued or running, or timeoutMs passes (CADR_ERROR_TIMEOUT). */
+int cadrServerDrain(cadrServer* server, uint32_t timeoutMs);
+
+/* Open-loop load: Poisson arrivals at requestsPerSecond for durationSeconds;
+ * hotFraction of the requests go uniformly to scenes [0, numHotScenes), the
+ * rest uniformly to the others; each request takes a view uniformly from
+ * views[0..numViews).  Completed requests are released at once. */
+typedef struct cadrLoadSpec {
+	double requestsPerSecond;
+	double durationSeconds;
+	uint64_t seed;
+	uint32_t numHotScenes;
+	float hotFraction;
+	const cadrView* views;          /* host */
+	uint32_t numViews;
+	uint32_t reserved;
+} cadrLoadSpec;
+
+typedef struct cadrLoadStats {
+	uint64_t submitted, completed, refused;
+	double elapsedSeconds;          /* first arrival to last completion          */
+	double requestsPerSecond;       /* completed / elapsed                       */
+	double drawablesPerSecond;      /* culled (drawable, view) tests / elapsed   */
+	double p50Ms, p99Ms, p999Ms, maxMs, meanMs;
+	uint64_t visible;
+	uint64_t launches;
+	double meanViewsPerLaunch;
+} cadrLoadStats;
+
+int cadrServerRunLoad(cadrServer* server, const cadrLoadSpec* spec, cadrLoadStats* stats);
+
 #ifdef __cplusplus
 }
 #endif
