@@ -43,7 +43,13 @@ def parse():
     p.add_argument("--batch-views", type=int, default=256)
     p.add_argument("--batch-steps", type=int, default=10)
     p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--no-uneven", action="store_true")
+    p.add_argument("--no-served", action="store_true")
+    p.add_argument("--served-rates", default="5000,10000,20000,50000", help="Poisson arrival rates (req/s)")
+    p.add_argument("--served-seconds", type=float, default=1.0, help="arrival window per rate")
+    p.add_argument("--served-budget-gib", type=float, default=64.0, help="output memory budget of the server")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-tune", action="store_true", help="use --block/--items instead of the autotuner's choice")
     p.add_argument("--scale", type=float, default=1.0, help="root-count scale of full10m (1.0 = 10M drawables)")
     return p.parse_args()
 
@@ -85,6 +91,7 @@ def main():
     from cadr_amd import _native as N
     from cadr_amd import cull, device, scenes
 
+    from cadr_amd.tune import autotune, describe
     device.set_device(0)
     stream = device.Stream()
     s = stream.handle
@@ -97,6 +104,12 @@ def main():
     config = cull.make_config(args.block, args.items)
     views1 = cull.make_views([scenes.DEFAULT_CAMERA])
     out1 = cull.CullOutputs(1, n, ds.K)
+    tuned1 = None
+    if not args.no_tune:
+        ds.propagate(s)
+        config, table = autotune([(ds, views1, out1)], single_view=True, reps=5, stream=s)
+        tuned1 = {"choice": describe(config), "candidates_ms": {tune_key(t): t["ms"] for t in table}}
+        log(f"single-view autotune: {describe(config)}  {tuned1['candidates_ms']}")
     batch1 = cull.CullBatch(config, [(ds, views1, out1)])
 
     lib = N.load()
@@ -175,7 +188,7 @@ def main():
         "config": {"workload": "full10m: 1000x100x100 transform tree, 10M drawables, 64 StateSets, 3 LODs, "
                                "1 view (propagation + cull + LOD + per-StateSet compaction)",
                    "drawables": n, "nodes": int(sum(l[0] for l in ds.levels)), "statesets": ds.K, "views": 1,
-                   "block": args.block, "items_per_thread": args.items},
+                   "block": config.blockSize, "items_per_thread": config.itemsPerThread, "autotune": tuned1},
         "roofline": {"bound": "hbm", "kernel": "propagateKernel<false,true> (leaf level + world spheres)",
                      "achieved": leaf_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": leaf_gbs / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes": leaf_bytes, "avg_ms": leaf_ms},
@@ -185,23 +198,37 @@ def main():
     log(f"full10m: {ms_per_step:.3f} ms/frame, leaf {leaf_ms:.3f} ms ({leaf_gbs:.0f} GB/s), "
         f"cull {cull_ms:.3f} ms, visible {survivors1}")
 
+    served_config = cull.make_config(args.block, args.items)
     if not args.no_batch:
-        result["batched"] = bench_batched(args, ds, config, stream, lib, N, cull, scenes, device)
+        result["batched"], served_config = bench_batched(args, ds, config, stream, lib, N, cull, scenes, device)
+    if not args.no_uneven or not args.no_served:
+        batch1.close(); out1.close(); ds.close()
+    if not args.no_uneven:
+        result["uneven"] = bench_uneven(args, config, stream, lib, N, cull, scenes, device)
+    if not args.no_served:
+        result["served"] = bench_served(args, served_config, N, cull, scenes, device)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)
 
 
-def bench_batched(args, ds, config, stream, lib, N, cull, scenes, device):
+def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
     """batch256: all views in one persistent launch (graph-replayed) vs the plain
-    pass launched once per view."""
+    pass (cadrCullView with the single-view config) launched once per view.
+    Returns (result, batched config)."""
+    from cadr_amd.tune import autotune, describe
     s = stream.handle
     V = args.batch_views
     cams = scenes.batch_cameras(V)
     views = cull.make_views(cams)
     n = ds.n
     out = cull.CullOutputs(V, n, ds.K)
+    config, tuned = cull.make_config(args.block, args.items), None
+    if not args.no_tune:
+        config, table = autotune([(ds, views, out)], reps=2, stream=s)
+        tuned = {"choice": describe(config), "candidates_ms": {tune_key(t): t["ms"] for t in table}}
+        log(f"batch{V} autotune: {describe(config)}  {tuned['candidates_ms']}")
     batch = cull.CullBatch(config, [(ds, views, out)])
     # plain: one single-view launch per view
     ws = cull.Workspace(max(batch.status_words, 1), 4)
@@ -217,7 +244,7 @@ def bench_batched(args, ds, config, stream, lib, N, cull, scenes, device):
 
     def run_plain():
         for j, dj in plain_jobs:
-            N.check(lib.cadrCullView(ctypes.byref(config), dj.address, N.ptr(j), ws.handle, s))
+            N.check(lib.cadrCullView(ctypes.byref(view_config), dj.address, N.ptr(j), ws.handle, s))
 
     for _ in range(2):
         run_plain()
@@ -263,6 +290,9 @@ def bench_batched(args, ds, config, stream, lib, N, cull, scenes, device):
         "roofline": {"bound": "hbm", "kernel": f"cullKernel<{config.blockSize},{config.itemsPerThread}>",
                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "algorithmic_bytes": alg_bytes},
+        "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags,
+                   "autotune": tuned},
+        "plain_config": {"block": view_config.blockSize, "items_per_thread": view_config.itemsPerThread},
         "plain_ms_per_frame": plain_ms,
         "plain_culled_drawables_per_s": tests / (plain_ms * 1e-3),
         "speedup_vs_plain": plain_ms / batch_ms,
@@ -274,7 +304,192 @@ def bench_batched(args, ds, config, stream, lib, N, cull, scenes, device):
     batch.close(); out.close(); ws.close()
     for _, dj in plain_jobs:
         dj.close()
+    return res, config
+
+
+def level_descs(N, ds):
+    """cadrTransformLevel of every level of a DeviceScene (the last one writes spheres)."""
+    out, prev = [], None
+    for d, (cnt, lb, pb, wb) in enumerate(ds.levels):
+        lv = np.zeros(1, dtype=N.LEVEL_DTYPE)
+        lv["localMatrices"], lv["worldMatrices"], lv["count"] = lb.address, wb.address, cnt
+        if prev is not None:
+            lv["parentIndex"], lv["parentWorld"], lv["parentCount"] = pb.address, prev[1].address, prev[0]
+        if d == len(ds.levels) - 1:
+            lv["meshIndex"], lv["meshes"], lv["numMeshes"], lv["spheres"] = (ds.mesh_index.address, ds.meshes.address,
+                                                                           ds.num_meshes, ds.spheres.address)
+        out.append(lv)
+        prev = (cnt, wb)
+    return out
+
+
+def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
+    """uneven64: 64 resident scenes (generated in HBM), each with 1..32 views;
+    a frame propagates every scene's tree and culls all views of all scenes in
+    one persistent launch, the whole frame captured in one HIP graph.  Against
+    it: the plain pass, one cadrCullView launch per (scene, view)."""
+    from cadr_amd.device import Event
+    from cadr_amd.tune import autotune, describe
+    s = stream.handle
+    t0 = time.time()
+    specs = scenes.uneven_specs()
+    dss, entries, views_l = [], [], []
+    for i, (n, depth, lods, nv) in enumerate(specs):
+        ds = cull.DeviceScene(scenes.uneven_synth(i, n, depth, lods, nv))
+        views = cull.make_views(ds.scene.cameras)
+        out = cull.CullOutputs(nv, n, ds.K)
+        dss.append(ds); views_l.append(views); entries.append((ds, views, out))
+    device.synchronize()
+    log(f"uneven64: {len(specs)} scenes, {sum(d.n for d in dss)} drawables, "
+        f"{sum(v.shape[0] for v in views_l)} views generated in {time.time() - t0:.1f}s")
+    levels = [level_descs(N, ds) for ds in dss]
+    for lvs in levels:
+        for lv in lvs:
+            N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
+    config, tuned = cull.make_config(args.block, args.items), None
+    if not args.no_tune:
+        config, table = autotune(entries, reps=2, stream=s)
+        tuned = {"choice": describe(config), "candidates_ms": {tune_key(t): t["ms"] for t in table}}
+        log(f"uneven64 autotune: {describe(config)}  {tuned['candidates_ms']}")
+    batch = cull.CullBatch(config, entries)
+
+    def propagate_all():
+        for lvs in levels:
+            for lv in lvs:
+                N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
+
+    # plain: one single-view launch per (scene, view), same outputs
+    ws = cull.Workspace(max(batch.status_words, 1), 4)
+    plain = []
+    for j, (ds, views, out) in enumerate(entries):
+        for v in range(views.shape[0]):
+            hj = batch.host_jobs[j:j + 1].copy()
+            hj["views"] = batch.host_jobs["views"][j] + v * N.VIEW_DTYPE.itemsize
+            hj["commands"] = out.commands.address + v * ds.n * 20
+            hj["counts"] = out.counts.address + v * ds.K * 4
+            hj["numViews"] = 1
+            plain.append((hj, device.DeviceBuffer.from_array(hj)))
+
+    def run_plain():
+        for hj, dj in plain:
+            N.check(lib.cadrCullView(ctypes.byref(view_config), dj.address, N.ptr(hj), ws.handle, s))
+
+    propagate_all()
+    run_plain()
+    stream.synchronize()
+    reps = 3
+    e0, e1 = Event(), Event()
+    e0.record(s)
+    for _ in range(reps):
+        run_plain()
+    e1.record(s)
+    stream.synchronize()
+    plain_ms = e0.elapsed_ms(e1) / reps
+    counts_plain = [e[2].download()[1] for e in entries]
+
+    frame = Graph()
+    frame.begin(s)
+    propagate_all()
+    batch.run(s)
+    frame.end(s)
+    cull_only = Graph()
+    cull_only.begin(s)
+    batch.run(s)
+    cull_only.end(s)
+    for g in (frame, cull_only):
+        g.launch(s)
+        g.launch(s)
+    stream.synchronize()
+    steps = max(3, args.batch_steps)
+    e0.record(s)
+    for _ in range(steps):
+        frame.launch(s)
+    e1.record(s)
+    stream.synchronize()
+    frame_ms = e0.elapsed_ms(e1) / steps
+    e0.record(s)
+    for _ in range(steps):
+        cull_only.launch(s)
+    e1.record(s)
+    stream.synchronize()
+    cull_ms = e0.elapsed_ms(e1) / steps
+    counts = [e[2].download()[1] for e in entries]
+    same = all(np.array_equal(a, b) for a, b in zip(counts, counts_plain))
+    survivors = int(sum(c.sum() for c in counts))
+    tests = sum(ds.n * v.shape[0] for ds, v in zip(dss, views_l))
+    nodes = sum(sum(l[0] for l in ds.levels) for ds in dss)
+    leaves = sum(ds.n for ds in dss)
+    cull_bytes = sum(ds.n * 20 for ds in dss) + survivors * 20 + sum(ds.K * 4 * v.shape[0] for ds, v in zip(dss, views_l))
+    prop_bytes = nodes * (64 + 64 + 4) + leaves * (4 + 16)
+    cull_gbs = cull_bytes / (cull_ms * 1e-3) / 1e9
+    res = {
+        "workload": "uneven64: 64 scenes of round(exp(U[ln 1e4, ln 5e6])) drawables, tree depth U{1..4}, LOD count "
+                    "U{1..4}, 64 StateSets, U{1..32} views each; propagation of every scene + all views of all "
+                    "scenes in one persistent cull launch, one HIP graph per frame",
+        "scenes": len(dss), "drawables": leaves, "nodes": nodes, "views": int(sum(v.shape[0] for v in views_l)),
+        "tests_per_frame": tests, "visible": survivors,
+        "ms_per_frame": frame_ms, "cull_ms": cull_ms,
+        "culled_drawables_per_s": tests / (frame_ms * 1e-3),
+        "cull_roofline": {"bound": "hbm", "kernel": f"cullKernel<{config.blockSize},{config.itemsPerThread}>",
+                          "achieved": cull_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": cull_gbs / HBM_PEAK_GBS,
+                          "algorithmic_bytes": cull_bytes},
+        "frame_hbm_frac": (cull_bytes + prop_bytes) / (frame_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags,
+                   "autotune": tuned},
+        "plain_cull_ms": plain_ms,
+        "speedup_cull_vs_plain": plain_ms / cull_ms,
+        "plain_and_batched_counts_identical": same,
+    }
+    log(f"uneven64: frame {frame_ms:.3f} ms, cull {cull_ms:.3f} ms ({cull_gbs:.0f} GB/s alg), "
+        f"plain cull {plain_ms:.3f} ms ({plain_ms / cull_ms:.1f}x), identical={same}")
+    frame.close(); cull_only.close()
+    for _, dj in plain:
+        dj.close()
+    batch.close(); ws.close()
+    for ds, _, out in entries:
+        out.close(); ds.close()
     return res
+
+
+def bench_served(args, config, N, cull, scenes, device):
+    """served128: 128 resident flat scenes (generated in HBM); Poisson cull
+    requests, 80% to the 16 hottest scenes, admitted by output memory and
+    batched across scenes by the serving loop; latency from scheduled arrival
+    to the result on the host."""
+    from cadr_amd.serve import Server
+    t0 = time.time()
+    sizes = scenes.served_specs()
+    dss = [cull.DeviceScene(scenes.served_synth(i, n)) for i, n in enumerate(sizes)]
+    device.synchronize()
+    views = cull.make_views(scenes.batch_cameras(256))
+    srv = Server(block=config.blockSize, items=config.itemsPerThread, flags=config.flags, max_batch_views=256, inflight=4,
+                 budget_bytes=int(args.served_budget_gib * (1 << 30)))
+    for ds in dss:
+        srv.add_scene(ds)
+    srv.start()
+    log(f"served128: {len(dss)} scenes, {sum(sizes)} drawables resident, set up in {time.time() - t0:.1f}s")
+    srv.run_load(views, rate=2000.0, duration=0.2, seed=99, hot=16, hot_fraction=0.8)  # warm-up
+    runs = []
+    for rate in [float(r) for r in args.served_rates.split(",") if r]:
+        st = srv.run_load(views, rate=rate, duration=args.served_seconds, seed=int(rate), hot=16, hot_fraction=0.8)
+        st["offered_rate"] = rate
+        runs.append(st)
+        log(f"served128 @ {rate:.0f} req/s: {st['requestsPerSecond']:.0f} req/s, "
+            f"{st['drawablesPerSecond'] / 1e9:.1f} G tests/s, p50 {st['p50Ms']:.3f} ms, p99 {st['p99Ms']:.3f} ms, "
+            f"p999 {st['p999Ms']:.3f} ms, {st['meanViewsPerLaunch']:.1f} views/launch")
+    srv.close()
+    for ds in dss:
+        ds.close()
+    return {"workload": "served128: 128 flat scenes of round(exp(U[ln 1e4, ln 2e7])) drawables (64 StateSets, 3 LODs), "
+                        "Poisson arrivals, 80% of requests to the 16 hottest scenes, views as batch256",
+            "drawables_resident": int(sum(sizes)), "output_budget_gib": args.served_budget_gib,
+            "max_batch_views": 256, "inflight_batches": 4,
+            "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags},
+            "runs": runs}
+
+
+def tune_key(t):
+    return f"{t['block']}x{t['items']}{'' if t['flags'] else '+ts'}" + ("" if t["agrees"] else "!mismatch")
 
 
 class Graph:

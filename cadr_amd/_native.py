@@ -51,6 +51,20 @@ LEVEL_DTYPE = np.dtype([("localMatrices", "<u8"), ("parentIndex", "<u8"), ("pare
 assert DRAWABLE_DTYPE.itemsize == 48 and MESH_DTYPE.itemsize == 64 and VIEW_DTYPE.itemsize == 96
 assert TILE_DTYPE.itemsize == 32 and JOB_DTYPE.itemsize == 80 and CMD_DTYPE.itemsize == 20
 assert LEVEL_DTYPE.itemsize == 72
+# cadrSynthLevel (include/cadr_synth.h)
+SYNTH_LEVEL_DTYPE = np.dtype([("seed", "<u8"), ("count", "<u8"), ("permuteKey", "<u8"), ("localMatrices", "<u8"),
+                              ("parentIndex", "<u8"), ("meshIndex", "<u8"), ("meshes", "<u8"), ("spheres", "<u8"),
+                              ("stream", "<u4"), ("transform", "<u4"), ("translationRange", "<f4"),
+                              ("fanout", "<u4"), ("meshStream", "<u4"), ("numMeshes", "<u4")])
+assert SYNTH_LEVEL_DTYPE.itemsize == 88
+# cadrCullTuneJob / cadrCullTuneResult (include/cadr_hip.h)
+TUNE_JOB_DTYPE = np.dtype([("job", JOB_DTYPE), ("stateSetCounts", "<u8")])
+TUNE_RESULT_DTYPE = np.dtype([("blockSize", "<u4"), ("itemsPerThread", "<u4"), ("workgroupsPerCU", "<u4"),
+                              ("flags", "<u4"), ("milliseconds", "<f4"), ("agrees", "<u4")])
+assert TUNE_JOB_DTYPE.itemsize == 88 and TUNE_RESULT_DTYPE.itemsize == 24
+TUNE_SINGLE_VIEW = 1
+SYNTH_ROTATION_TRANSLATION = 0
+SYNTH_SCALE_TRANSLATION = 1
 
 TILE_FIRST = 0x80000000
 TILE_LAST = 0x40000000
@@ -74,6 +88,35 @@ class PushConstants(ctypes.Structure):
 class CullConfig(ctypes.Structure):
     _fields_ = [("blockSize", c_u32), ("itemsPerThread", c_u32), ("workgroupsPerCU", c_u32),
                 ("flags", c_u32)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("cull", CullConfig), ("maxBatchViews", c_u32), ("maxInflightBatches", c_u32),
+                ("outputBudgetBytes", c_u64), ("maxQueuedRequests", c_u32), ("reserved", c_u32)]
+
+
+class ServerScene(ctypes.Structure):
+    _fields_ = [("spheres", c_u64), ("meshIndex", c_u64), ("meshes", c_u64), ("stateSetCounts", c_void_p),
+                ("numStateSets", c_u32), ("numMeshes", c_u32), ("numDrawables", c_u32), ("reserved", c_u32)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("requestId", c_u64), ("sceneId", c_u32), ("numStateSets", c_u32), ("commands", c_u64),
+                ("counts", c_u64), ("visible", c_u64), ("submitNs", c_u64), ("completeNs", c_u64)]
+
+
+class LoadSpec(ctypes.Structure):
+    _fields_ = [("requestsPerSecond", ctypes.c_double), ("durationSeconds", ctypes.c_double), ("seed", c_u64),
+                ("numHotScenes", c_u32), ("hotFraction", c_float), ("views", c_void_p), ("numViews", c_u32),
+                ("reserved", c_u32)]
+
+
+class LoadStats(ctypes.Structure):
+    _fields_ = [("submitted", c_u64), ("completed", c_u64), ("refused", c_u64), ("elapsedSeconds", ctypes.c_double),
+                ("requestsPerSecond", ctypes.c_double), ("drawablesPerSecond", ctypes.c_double),
+                ("p50Ms", ctypes.c_double), ("p99Ms", ctypes.c_double), ("p999Ms", ctypes.c_double),
+                ("maxMs", ctypes.c_double), ("meanMs", ctypes.c_double), ("visible", c_u64), ("launches", c_u64),
+                ("meanViewsPerLaunch", ctypes.c_double)]
 
 
 class CadrError(RuntimeError):
@@ -120,6 +163,20 @@ SIGNATURES = {
     "cadrCullBatch": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_u32, c_void_p, c_void_p]),
     "cadrCullView": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_void_p, c_void_p]),
     "cadrCullTileSize": (c_u32, [ctypes.POINTER(CullConfig)]),
+    "cadrServerCreate": (c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(c_void_p)]),
+    "cadrServerDestroy": (c_int, [c_void_p]),
+    "cadrServerAddScene": (c_int, [c_void_p, ctypes.POINTER(ServerScene), ctypes.POINTER(c_u32)]),
+    "cadrServerStart": (c_int, [c_void_p]),
+    "cadrServerSubmit": (c_int, [c_void_p, c_u32, c_void_p, ctypes.POINTER(c_u64)]),
+    "cadrServerPoll": (c_int, [c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
+    "cadrServerCopyCounts": (c_int, [c_void_p, c_u64, c_void_p, c_u32]),
+    "cadrServerRelease": (c_int, [c_void_p, c_u64]),
+    "cadrServerDrain": (c_int, [c_void_p, c_u32]),
+    "cadrServerRunLoad": (c_int, [c_void_p, ctypes.POINTER(LoadSpec), ctypes.POINTER(LoadStats)]),
+    "cadrCullAutotune": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_void_p, c_u32, c_void_p, c_u32,
+                                 ctypes.POINTER(c_u32), ctypes.POINTER(CullConfig), c_void_p]),
+    "cadrSynthesizeLevel": (c_int, [c_void_p, c_void_p]),
+    "cadrSynthPermute": (c_u64, [c_u64, c_u64, c_u64]),
 }
 
 _lib = None

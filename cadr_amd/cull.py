@@ -56,17 +56,25 @@ class DeviceScene:
     """A scene resident in HBM: transform levels, world spheres, mesh table."""
 
     def __init__(self, scene, keep_levels: bool = True, chunk: int = 1 << 18):
-        """scene: a Scene (host arrays) or a TreeSpec (streamed level by level in chunks)."""
+        """scene: a Scene (host arrays), a TreeSpec (streamed level by level in
+        chunks) or a SynthTree / SynthFlat (generated in device memory)."""
         self.scene = scene
         self.n = scene.num_drawables
         self.K = int(len(scene.stateset_counts))
         self.stateset_counts = np.asarray(scene.stateset_counts, dtype=np.uint32)
         self.num_meshes = int(scene.meshes.shape[0])
         self.meshes = DeviceBuffer.from_array(scene.meshes)
-        self.mesh_index = DeviceBuffer.from_array(scene.mesh_index)
         self.spheres = DeviceBuffer(max(16 * self.n, 16))
         self.levels = []
-        if keep_levels and isinstance(scene, Scene):
+        synth = getattr(scene, "synth", False)
+        if synth:
+            self.mesh_index = DeviceBuffer(max(4 * self.n, 16))
+            self._synthesize(scene)
+        else:
+            self.mesh_index = DeviceBuffer.from_array(scene.mesh_index)
+        if synth:
+            pass
+        elif keep_levels and isinstance(scene, Scene):
             for local, parent in scene.levels:
                 cnt = local.shape[0]
                 lb = DeviceBuffer.from_array(local)
@@ -86,6 +94,28 @@ class DeviceScene:
                         pb.upload(parent, offset=4 * c0)
                 self.levels.append((cnt, lb, pb, wb))
         self._tiles = {}
+
+    def _synthesize(self, spec) -> None:
+        """Generates the scene's levels (or, for a flat scene, its world
+        spheres) in device memory with cadrSynthesizeLevel."""
+        lib = N.load()
+        if spec.flat:
+            L = spec.level_desc()
+            L["meshIndex"], L["meshes"], L["spheres"] = self.mesh_index.address, self.meshes.address, self.spheres.address
+            N.check(lib.cadrSynthesizeLevel(N.ptr(L), None))
+            return
+        for d, cnt in enumerate(spec.sizes):
+            lb = DeviceBuffer(max(64 * cnt, 16))
+            wb = DeviceBuffer(max(64 * cnt, 16))
+            pb = DeviceBuffer(max(4 * cnt, 16)) if d > 0 else None
+            L = spec.level_desc(d)
+            L["localMatrices"] = lb.address
+            if pb is not None:
+                L["parentIndex"] = pb.address
+            if d == len(spec.sizes) - 1:
+                L["meshIndex"] = self.mesh_index.address
+            N.check(lib.cadrSynthesizeLevel(N.ptr(L), None))
+            self.levels.append((cnt, lb, pb, wb))
 
     def tiles(self, tile: int):
         if tile not in self._tiles:

@@ -340,6 +340,96 @@ def uneven_scene(index: int, drawables: int, depth: int, lod_count: int, num_vie
                  counts, cams)
 
 
+def balanced_counts(n: int, k: int) -> np.ndarray:
+    """StateSet sizes floor((j+1)n/k) - floor(jn/k): positions split into k near-equal ranges."""
+    edges = (np.arange(k + 1, dtype=np.int64) * n) // k
+    return np.diff(edges).astype(np.uint32)
+
+
+class SynthTree:
+    """A transform tree generated in device memory by cadrSynthesizeLevel.
+
+    Level sizes as uneven_scene: `depth` levels with equal fan-out
+    f = ceil(drawables^(1/depth)).  Inner node i is tree node i with parent
+    i // f; the drawable at leaf position p is tree node perm(p) (the
+    generator's Feistel permutation, key = seed), so the near-equal StateSet
+    ranges of positions hold uniformly random sets of leaves.  Matrices are
+    rotation_translation(seed, 10 + d, node), mesh indexes rand_int(seed, 3,
+    node) over the 64 meshes."""
+    synth = True
+
+    def __init__(self, name: str, drawables: int, depth: int, lod_count: int, seed: int, num_statesets: int = 64,
+                 cameras=None, trange: float = 50.0, num_meshes: int = 64):
+        self.name, self.seed, self.trange = name, seed, trange
+        f = max(1, int(math.ceil(drawables ** (1.0 / depth) - 1e-9)))
+        sizes = [drawables]
+        for _ in range(depth - 1):
+            sizes.append((sizes[-1] + f - 1) // f)
+        self.sizes = sizes[::-1]
+        self.fanout = f
+        self.meshes = make_meshes(seed, num_meshes, lod_count)
+        self.stateset_counts = balanced_counts(drawables, num_statesets)
+        self.cameras = list(cameras or [])
+        self.flat = False
+
+    @property
+    def num_drawables(self) -> int:
+        return self.sizes[-1]
+
+    def level_desc(self, d: int) -> np.ndarray:
+        """cadrSynthLevel of level d without output addresses."""
+        from ._native import SYNTH_LEVEL_DTYPE, SYNTH_ROTATION_TRANSLATION
+        L = np.zeros(1, dtype=SYNTH_LEVEL_DTYPE)
+        L["seed"], L["count"], L["stream"] = self.seed, self.sizes[d], 10 + d
+        L["transform"], L["translationRange"] = SYNTH_ROTATION_TRANSLATION, self.trange
+        L["fanout"] = self.fanout if d > 0 else 0
+        if d == len(self.sizes) - 1:
+            L["permuteKey"] = self.seed
+            L["meshStream"], L["numMeshes"] = 3, self.meshes.shape[0]
+        return L
+
+
+class SynthFlat:
+    """A flat scene generated in device memory: the drawables' world spheres
+    directly (plain100k transforms, scale_translation(seed, 4, i) with
+    translation range `trange`), mesh indexes rand_int(seed, 3, i); StateSets
+    are near-equal ranges of the i.i.d. drawables."""
+    synth = True
+
+    def __init__(self, name: str, drawables: int, seed: int, lod_count: int = 3, num_statesets: int = 64,
+                 trange: float = 500.0, num_meshes: int = 64):
+        self.name, self.seed, self.trange = name, seed, trange
+        self.sizes = [drawables]
+        self.meshes = make_meshes(seed, num_meshes, lod_count)
+        self.stateset_counts = balanced_counts(drawables, num_statesets)
+        self.cameras = []
+        self.flat = True
+
+    @property
+    def num_drawables(self) -> int:
+        return self.sizes[0]
+
+    def level_desc(self) -> np.ndarray:
+        from ._native import SYNTH_LEVEL_DTYPE, SYNTH_SCALE_TRANSLATION
+        L = np.zeros(1, dtype=SYNTH_LEVEL_DTYPE)
+        L["seed"], L["count"], L["stream"] = self.seed, self.sizes[0], 4
+        L["transform"], L["translationRange"] = SYNTH_SCALE_TRANSLATION, self.trange
+        L["meshStream"], L["numMeshes"] = 3, self.meshes.shape[0]
+        return L
+
+
+def uneven_synth(index: int, drawables: int, depth: int, lod_count: int, num_views: int,
+                 seed: int = SEED_BASE + 3) -> SynthTree:
+    """Scene `index` of uneven64, generated in device memory."""
+    sseed = seed + 0x1000 * (index + 1)
+    return SynthTree(f"uneven{index}", drawables, depth, lod_count, sseed,
+                     cameras=random_cameras(sseed, 20, 0, num_views))
+
+
+def served_synth(index: int, drawables: int, seed: int = SEED_BASE + 4) -> SynthFlat:
+    return SynthFlat(f"served{index}", drawables, seed + 0x1000 * (index + 1))
+
+
 def served_specs(num_scenes: int = 128, lo: float = 1e4, hi: float = 2e7, budget_drawables: int | None = None,
                  seed: int = SEED_BASE + 4):
     u = rand_unit(seed, 40, 0, num_scenes, 1)[:, 0].astype(np.float64)

@@ -346,6 +346,35 @@ int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCul
 /* Kernel tile size for a config (blockSize * itemsPerThread), 0 if unsupported. */
 uint32_t cadrCullTileSize(const cadrCullConfig* config);
 
+/* Autotuner of the cull launch shape (workgroup size, drawables per thread,
+ * hence the scan tile, tile-sphere classification, workgroups per CU).
+ * Times every candidate on the caller's own jobs — real launches that write
+ * the jobs' commands and counts — with HIP events over `reps` launches after
+ * one warm-up launch, and returns the fastest in *best.  Tiles are built
+ * per candidate tile size from stateSetCounts (the jobs' `tiles` fields are
+ * ignored).  With candidates == NULL every supported shape is tried (with
+ * and without tile spheres for batched launches).  results (optional)
+ * receives one entry per candidate; `agrees` is 1 when the candidate's
+ * counts equal those of the first candidate that ran.
+ * CADR_TUNE_SINGLE_VIEW times cadrCullView (one job with one view).
+ * No counterpart in the reference. */
+typedef struct cadrCullTuneJob {
+	cadrCullJob job;                /* tiles ignored                          */
+	const uint32_t* stateSetCounts; /* host, job.numStateSets entries         */
+} cadrCullTuneJob;
+
+typedef struct cadrCullTuneResult {
+	cadrCullConfig config;
+	float milliseconds;             /* mean per launch; +inf if it failed     */
+	uint32_t agrees;
+} cadrCullTuneResult;
+#define CADR_TUNE_SINGLE_VIEW 1u
+
+int cadrCullAutotune(const cadrCullTuneJob* jobs, uint32_t numJobs, uint32_t flags, uint32_t reps,
+                     const cadrCullConfig* candidates, uint32_t numCandidates,
+                     cadrCullTuneResult* results, uint32_t resultCapacity, uint32_t* numResults,
+                     cadrCullConfig* best, cadrStream stream);
+
 /* ------------------------------------------------------------------------ */
 /* Serving: scenes resident in HBM, cull requests (scene, view) admitted by
  * output memory and batch slots, and batched continuously — every launch
