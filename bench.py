@@ -5,9 +5,18 @@ Default run (one GPU): the full10m workload — a 3-level transform tree of
 10,101,000 nodes (1,000 roots -> 100 children -> 100 leaves), 10M leaf
 drawables in 64 StateSets, 3 LODs — propagated level by level to world
 matrices and world spheres, then frustum-culled, LOD-selected and compacted
-per StateSet for one view.  A step is one such frame.  The line also reports
-the batch256 workload (the same scene culled for 256 views in one persistent
-launch, replayed from a HIP graph) against the plain pass run once per view.
+per StateSet for one view.  A step is one such frame; `value` is drawables
+culled per second over the timed frames.
+
+Further legs, reported beside it:
+  batched  batch256: the full10m scene culled for 256 views in one persistent
+           launch replayed from a HIP graph, against the plain pass launched
+           once per view;
+  uneven   uneven64: 64 scenes (10k..5M drawables, depth 1..4, 1..32 views),
+           every tree propagated and every view culled in one graph per frame;
+  served   served128: 128 resident scenes under Poisson cull requests through
+           the serving loop, p50/p99/p999 latency and throughput per rate.
+Kernel shapes come from the autotuner (cadrCullAutotune) unless --no-tune.
 
 Prints one JSON line (metric / value / roofline / cpu_baseline ...).
 """
@@ -38,8 +47,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--block", type=int, default=256)
-    p.add_argument("--items", type=int, default=4)
+    p.add_argument("--block", type=int, default=512, help="batched cull workgroup size (with --no-tune)")
+    p.add_argument("--items", type=int, default=4, help="batched cull drawables per thread (with --no-tune)")
+    p.add_argument("--view-items", type=int, default=16, help="single-view cull drawables per thread (with --no-tune)")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                   help="also copy BYTES device-to-device 3 times (reference kernel for HBM counter scaling)")
     p.add_argument("--batch-views", type=int, default=256)
     p.add_argument("--batch-steps", type=int, default=10)
     p.add_argument("--no-batch", action="store_true")
@@ -101,7 +113,7 @@ def main():
     ds = cull.DeviceScene(spec)  # streamed to HBM level by level
     log(f"scene generated and uploaded in {time.time() - t0:.1f}s: {ds.n} drawables, {sum(spec.sizes)} nodes")
     n = ds.n
-    config = cull.make_config(args.block, args.items)
+    config = cull.make_config(256, args.view_items)
     views1 = cull.make_views([scenes.DEFAULT_CAMERA])
     out1 = cull.CullOutputs(1, n, ds.K)
     tuned1 = None
@@ -208,6 +220,8 @@ def main():
     if not args.no_served:
         result["served"] = bench_served(args, served_config, N, cull, scenes, device)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, stream, lib, N, device)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)
@@ -486,6 +500,19 @@ def bench_served(args, config, N, cull, scenes, device):
             "max_batch_views": 256, "inflight_batches": 4,
             "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags},
             "runs": runs}
+
+
+def copy_calibration(nbytes, stream, lib, N, device):
+    """A plain device-to-device copy of nbytes, 3 times: under rocprofv3 --pmc
+    its FETCH_SIZE / WRITE_SIZE per dispatch against nbytes gives the scale
+    factor of those counters."""
+    src, dst = device.DeviceBuffer(nbytes), device.DeviceBuffer(nbytes)
+    N.check(lib.cadrFillU32(src.address, 0x3F800000, nbytes, stream.handle))
+    for _ in range(3):
+        N.check(lib.cadrMemcpyDtoD(dst.address, src.address, nbytes, stream.handle))
+    stream.synchronize()
+    log(f"copy calibration: 3 x {nbytes} bytes device to device")
+    src.close(); dst.close()
 
 
 def tune_key(t):

@@ -35,6 +35,8 @@
 #include "internal.h"
 
 #include <algorithm>
+#include <cstddef>
+#include <type_traits>
 #include <vector>
 
 namespace cadr {
@@ -61,14 +63,38 @@ struct CullParams {
 	uint32_t flags;
 };
 
-// Pointers read from job descriptors are generic; loads through the global
-// address space become global_load (not flat) and leave the LDS counter alone.
+// Addresses read from descriptors would be generic (FLAT) pointers, and a
+// FLAT access counts on both the vector-memory and the LDS counters: every
+// LDS wait would then also wait for outstanding output stores.  All device
+// memory is therefore accessed through global (addrspace 1) pointers, and
+// read-only descriptors (jobs, tiles, views, offsets) through constant
+// (addrspace 4) pointers, which become scalar loads when the address is
+// uniform.
 template <class T>
 using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+using gmut = __attribute__((address_space(1))) T*;
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
 template <class T>
 __device__ __forceinline__ gptr<T> globalPtr(uint64_t a)
 {
 	return reinterpret_cast<gptr<T>>(a);
+}
+template <class T>
+__device__ __forceinline__ gmut<T> globalMut(uint64_t a)
+{
+	return reinterpret_cast<gmut<T>>(a);
+}
+template <class T>
+__device__ __forceinline__ cptr<T> constPtr(uint64_t a)
+{
+	return reinterpret_cast<cptr<T>>(a);
+}
+template <class T>
+__device__ __forceinline__ uint64_t addr(const T* p)
+{
+	return reinterpret_cast<uint64_t>(p);
 }
 
 struct TileDesc {
@@ -102,19 +128,33 @@ __device__ __forceinline__ void loadTileItems(gptr<float4> spheres, gptr<uint32_
 	}
 }
 
+// Job descriptor j, field by field through a constant pointer (scalar loads).
+static_assert(offsetof(cadrCullJob, numDrawables) == 56 && sizeof(cadrCullJob) == 80, "cadrCullJob layout");
+__device__ __forceinline__ cadrCullJob loadJob(const cadrCullJob* jobs, uint32_t j)
+{
+	const uint64_t a = addr(jobs) + uint64_t(j) * sizeof(cadrCullJob);
+	const cptr<uint64_t> q = constPtr<uint64_t>(a);
+	const cptr<uint32_t> w = constPtr<uint32_t>(a + 56);
+	cadrCullJob J;
+	J.spheres = q[0]; J.meshIndex = q[1]; J.meshes = q[2]; J.tiles = q[3];
+	J.views = q[4]; J.commands = q[5]; J.counts = q[6];
+	J.numDrawables = w[0]; J.numTiles = w[1]; J.numStateSets = w[2];
+	J.numViews = w[3]; J.commandStride = w[4]; J.numMeshes = w[5];
+	return J;
+}
+
 __device__ __forceinline__ TileDesc loadTile(const cadrCullJob& J, uint32_t t, uint32_t tileSize)
 {
-	const uint4* p = reinterpret_cast<const uint4*>(J.tiles) + 2 * uint64_t(t);
-	const uint4 a = p[0];
-	const uint4 b = p[1];
+	const cptr<uint32_t> p = constPtr<uint32_t>(J.tiles) + 8 * uint64_t(t);
+	const uint32_t flags = p[3];
 	TileDesc d;
-	d.start = a.x;
-	d.count = a.y;
-	d.segBase = a.z;
-	d.k = a.w & CADR_TILE_STATESET_MASK;
-	d.first = (a.w & CADR_TILE_FIRST) != 0u;
-	d.last = (a.w & CADR_TILE_LAST) != 0u;
-	d.chain = b.x;
+	d.start = p[0];
+	d.count = p[1];
+	d.segBase = p[2];
+	d.k = flags & CADR_TILE_STATESET_MASK;
+	d.first = (flags & CADR_TILE_FIRST) != 0u;
+	d.last = (flags & CADR_TILE_LAST) != 0u;
+	d.chain = p[4];
 	if (d.chain >= J.numTiles) d.chain = t;  // malformed list: keep status accesses in range
 	if (d.start >= J.numDrawables) d.count = 0;
 	else if (d.count > J.numDrawables - d.start) d.count = J.numDrawables - d.start;
@@ -122,25 +162,24 @@ __device__ __forceinline__ TileDesc loadTile(const cadrCullJob& J, uint32_t t, u
 	return d;
 }
 
-// Per-view constants, loaded with scalar loads (the view index is uniform).
+// Per-view constants: scalar loads when the view index is uniform, vector
+// loads when each lane takes its own view.
 struct ViewC {
 	float4 r0, r1, r2;
 	float nearZ, farZ, sx0, sx1, sy0, sy1, lodScale, th0, th1, th2;
 };
 
-__device__ __forceinline__ ViewC loadView(const cadrView* __restrict__ views, uint32_t v)
+__device__ __forceinline__ ViewC loadView(uint64_t views, uint32_t v)
 {
-	const float4* p = reinterpret_cast<const float4*>(views + v);
+	// cadrView: viewRow[3][4], nearZ, farZ, sideX[2], sideY[2], lodScale, lodThreshold[3]
+	const cptr<float> p = constPtr<float>(views + uint64_t(v) * sizeof(cadrView));
 	ViewC c;
-	c.r0 = p[0];
-	c.r1 = p[1];
-	c.r2 = p[2];
-	const float4 a = p[3];  // nearZ, farZ, sideX[0], sideX[1]
-	const float4 b = p[4];  // sideY[0], sideY[1], lodScale, lodThreshold[0]
-	const float4 d = p[5];  // lodThreshold[1], lodThreshold[2], reserved
-	c.nearZ = a.x; c.farZ = a.y; c.sx0 = a.z; c.sx1 = a.w;
-	c.sy0 = b.x; c.sy1 = b.y; c.lodScale = b.z; c.th0 = b.w;
-	c.th1 = d.x; c.th2 = d.y;
+	c.r0 = make_float4(p[0], p[1], p[2], p[3]);
+	c.r1 = make_float4(p[4], p[5], p[6], p[7]);
+	c.r2 = make_float4(p[8], p[9], p[10], p[11]);
+	c.nearZ = p[12]; c.farZ = p[13]; c.sx0 = p[14]; c.sx1 = p[15];
+	c.sy0 = p[16]; c.sy1 = p[17]; c.lodScale = p[18]; c.th0 = p[19];
+	c.th1 = p[20]; c.th2 = p[21];
 	return c;
 }
 
@@ -293,27 +332,58 @@ __device__ uint32_t lookBackWave(const uint64_t* chain, uint32_t c, uint32_t epo
 	return excl;
 }
 
+// The mesh table of a job: an LDS copy when it is small, else device memory.
+struct MeshTable {
+	const cadrMesh* lds;
+	gptr<cadrMesh> mem;
+	bool inLds;
+};
+
+// Command fields of mesh m at the LOD the view selects for a sphere of radius
+// r at view depth vz.  LDS is a compile-time choice: a run-time branch
+// between the two sources would make the compiler wait for every
+// outstanding vector-memory operation (output stores included) where the
+// branches join.
+template <bool LDS>
+__device__ __forceinline__ void meshCommand(const MeshTable& M, uint32_t m, float r, float vz, const ViewC& V,
+                                            uint32_t& count, uint32_t& first, uint32_t& vofs)
+{
+	if constexpr (LDS) {
+		const cadrMesh& mesh = M.lds[m];
+		const uint32_t lod = lodFromDepth(r, vz, V, mesh.lodCount);
+		count = mesh.indexCount[lod];
+		first = mesh.firstIndex[lod];
+		vofs = uint32_t(mesh.vertexOffset);
+	} else {
+		const gptr<cadrMesh> mesh = M.mem + m;
+		const uint32_t lod = lodFromDepth(r, vz, V, mesh->lodCount);
+		count = mesh->indexCount[lod];
+		first = mesh->firstIndex[lod];
+		vofs = uint32_t(mesh->vertexOffset);
+	}
+}
+
 // A wave's compacted commands are staged in LDS (5 dwords each, a stride
 // that is conflict-free across the LDS banks) and then stored as runs of 64
 // consecutive dwords, so every store instruction writes 256 contiguous bytes
 // and HBM sees whole lines instead of partial ones.
-__device__ __forceinline__ void stageCommand(uint32_t* stage, uint32_t slot, const cadrMesh& mesh, uint32_t lod,
-                                             uint32_t drawable)
+__device__ __forceinline__ void stageCommand(uint32_t* stage, uint32_t slot, uint32_t count, uint32_t first,
+                                             uint32_t vofs, uint32_t drawable)
 {
 	uint32_t* o = stage + slot * 5u;
-	o[0] = mesh.indexCount[lod];
+	o[0] = count;
 	o[1] = 1u;
-	o[2] = mesh.firstIndex[lod];
-	o[3] = uint32_t(mesh.vertexOffset);
+	o[2] = first;
+	o[3] = vofs;
 	o[4] = drawable;
 }
 
 // Stores n staged commands (n <= 64) to commands[first .. first+n), skipping
 // slots at or beyond the stride.
-__device__ __forceinline__ void flushCommands(uint32_t* out, const uint32_t* stage, uint32_t first, uint32_t n,
+__device__ __forceinline__ void flushCommands(gmut<uint32_t> out, const uint32_t* stage, uint32_t first, uint32_t n,
                                               uint32_t stride, uint32_t lane)
 {
-	uint32_t* dst = out + uint64_t(first) * 5ull;
+	const gmut<uint32_t> dst = out + uint64_t(first) * 5ull;
 #pragma unroll
 	for (uint32_t k = 0; k < 5; k++) {
 		const uint32_t d = k * 64u + lane;
@@ -327,22 +397,22 @@ template <int BLOCK>
 __device__ __forceinline__ bool stageMeshes(cadrMesh* sMeshes, const cadrCullJob& J, uint32_t tid)
 {
 	if (J.numMeshes > kLdsMeshes) return false;
-	const float4* src = reinterpret_cast<const float4*>(J.meshes);
-	float4* dst = reinterpret_cast<float4*>(sMeshes);
-	for (uint32_t i = tid; i < J.numMeshes * 4u; i += BLOCK) dst[i] = src[i];
+	const gptr<uint32_t> src = globalPtr<uint32_t>(J.meshes);
+	uint32_t* dst = reinterpret_cast<uint32_t*>(sMeshes);
+	for (uint32_t i = tid; i < J.numMeshes * 16u; i += BLOCK) dst[i] = src[i];
 	return true;
 }
 
 // ---------------------------------------------------------------- one view
 
-template <int ITEMS>
+template <int ITEMS, bool LDS>
 __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCullJob& J, uint32_t t, uint32_t epoch,
-                                             const cadrMesh* meshes, uint32_t (*sCnt)[ITEMS], uint32_t* sExclP,
+                                             const MeshTable& meshes, uint32_t (*sCnt)[ITEMS], uint32_t* sExclP,
                                              uint32_t* sStage, uint32_t tid, uint32_t lane, uint32_t wave)
 {
 	constexpr int BLOCK = 256, WAVES = 4, WAVE_ITEMS = 64 * ITEMS;
 	const TileDesc T = loadTile(J, t, BLOCK * ITEMS);
-	const ViewC V = loadView(reinterpret_cast<const cadrView*>(J.views), 0);
+	const ViewC V = loadView(J.views, 0);
 	const gptr<float4> spheres = globalPtr<float4>(J.spheres);
 	const gptr<uint32_t> meshIndex = globalPtr<uint32_t>(J.meshIndex);
 
@@ -378,7 +448,7 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 		}
 		if (lane == 0) {
 			*sExclP = excl;
-			if (T.last && T.k < J.numStateSets) reinterpret_cast<uint32_t*>(J.counts)[T.k] = excl + total;
+			if (T.last && T.k < J.numStateSets) globalMut<uint32_t>(J.counts)[T.k] = excl + total;
 		}
 	}
 	__syncthreads();
@@ -387,16 +457,16 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 	for (uint32_t q = 0; q < wave; q++)
 		for (int j = 0; j < ITEMS; j++) pos += sCnt[q][j];
 	pos = uniform(pos);
-	uint32_t* out = reinterpret_cast<uint32_t*>(J.commands);
+	const gmut<uint32_t> out = globalMut<uint32_t>(J.commands);
 	uint32_t* stage = sStage + wave * 320u;
 #pragma unroll
 	for (int j = 0; j < ITEMS; j++) {
 		const uint32_t n = uint32_t(__builtin_popcountll(mask[j]));
 		if (n == 0) continue;
 		if ((mask[j] >> lane) & 1ull) {
-			const cadrMesh& mesh = meshes[mi[j]];
-			const uint32_t lod = lodFromDepth(sp[j].w, vz[j], V, mesh.lodCount);
-			stageCommand(stage, maskRank(mask[j]), mesh, lod, T.start + wave * WAVE_ITEMS + j * 64 + lane);
+			uint32_t count, first, vofs;
+			meshCommand<LDS>(meshes, mi[j], sp[j].w, vz[j], V, count, first, vofs);
+			stageCommand(stage, maskRank(mask[j]), count, first, vofs, T.start + wave * WAVE_ITEMS + j * 64 + lane);
 		}
 		waveSync();
 		flushCommands(out, stage, pos, n, J.commandStride, lane);
@@ -419,12 +489,17 @@ __global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
 	const uint32_t tid = threadIdx.x, lane = laneId(), wave = tid / 64;
 	const uint32_t epoch = __hip_atomic_load(&P.ctl[kCtlEpoch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	if (tid == 0) sTicket = atomicAdd(&P.ctl[kCtlTicket], 1u);
-	const cadrCullJob J = P.jobs[0];
+	const cadrCullJob J = loadJob(P.jobs, 0);
 	const bool ldsMeshes = stageMeshes<BLOCK>(sMeshes, J, tid);
 	__syncthreads();
 	const uint32_t t = uniform(sTicket);
-	if (t < P.totalWork) cullViewTile<ITEMS>(P, J, t, epoch, ldsMeshes ? sMeshes : reinterpret_cast<const cadrMesh*>(J.meshes),
-	                                         sCnt, &sExcl, sStage, tid, lane, wave);
+	const MeshTable meshes{sMeshes, globalPtr<cadrMesh>(J.meshes), ldsMeshes};
+	if (t < P.totalWork) {
+		if (ldsMeshes)
+			cullViewTile<ITEMS, true>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, tid, lane, wave);
+		else
+			cullViewTile<ITEMS, false>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, tid, lane, wave);
+	}
 
 	// the last workgroup to finish resets the ticket and starts a new epoch
 	__syncthreads();
@@ -483,12 +558,12 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			uint32_t lo = job, hi = P.numJobs - 1;
 			while (lo < hi) {
 				const uint32_t mid = (lo + hi + 1) / 2;
-				if (P.firstWork[mid] <= w) lo = mid; else hi = mid - 1;
+				if (constPtr<uint32_t>(addr(P.firstWork))[mid] <= w) lo = mid; else hi = mid - 1;
 			}
 			job = uniform(lo);
 		}
-		const cadrCullJob J = P.jobs[job];
-		const uint32_t t = w - P.firstWork[job];
+		const cadrCullJob J = loadJob(P.jobs, job);
+		const uint32_t t = w - constPtr<uint32_t>(addr(P.firstWork))[job];
 		const TileDesc T = loadTile(J, t, S::TILE);
 		const bool ldsMeshes = stageMeshes<BLOCK>(sh.meshes, J, tid);
 
@@ -569,9 +644,9 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		}
 		const bool emptyTile = T.count == 0;
 
-		const cadrView* __restrict__ views = reinterpret_cast<const cadrView*>(J.views);
-		uint64_t* chain0 = P.status + P.statusBase[job];
-		const cadrMesh* meshes = ldsMeshes ? sh.meshes : reinterpret_cast<const cadrMesh*>(J.meshes);
+		const uint64_t views = J.views;
+		uint64_t* chain0 = P.status + constPtr<uint64_t>(addr(P.statusBase))[job];
+		const MeshTable meshes{sh.meshes, globalPtr<cadrMesh>(J.meshes), ldsMeshes};
 
 		for (uint32_t g0 = 0; g0 < J.numViews; g0 += kViewGroup) {
 			const uint32_t vg = uniform(min(kViewGroup, J.numViews - g0));
@@ -618,41 +693,48 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 				}
 				sh.excl[lane] = excl;
 				if (T.last && T.k < J.numStateSets)
-					reinterpret_cast<uint32_t*>(J.counts)[uint64_t(g0 + lane) * J.numStateSets + T.k] = excl + total;
+					globalMut<uint32_t>(J.counts)[uint64_t(g0 + lane) * J.numStateSets + T.k] = excl + total;
 			}
 			__syncthreads();
 
 			// 3. compacted writes: lane r of the wave stores its r-th visible drawable
-			for (uint32_t v = 0; v < vg; v++) {
-				const uint32_t run = uniform(sh.cnt[v][wave]);
-				if (run == 0) continue;
-				uint32_t wavePos = sh.excl[v];
-				for (uint32_t q = 0; q < wave; q++) wavePos += sh.cnt[v][q];
-				wavePos = uniform(wavePos);
-				uint32_t r0 = 0;
+			auto writeViews = [&](auto lds) {
+				for (uint32_t v = 0; v < vg; v++) {
+					const uint32_t run = uniform(sh.cnt[v][wave]);
+					if (run == 0) continue;
+					uint32_t wavePos = sh.excl[v];
+					for (uint32_t q = 0; q < wave; q++) wavePos += sh.cnt[v][q];
+					wavePos = uniform(wavePos);
+					uint32_t r0 = 0;
 #pragma unroll
-				for (int j = 0; j < ITEMS; j++) {
-					const uint64_t m = sh.mask[v][wave][j];
-					if ((m >> lane) & 1ull) sh.list[wave][r0 + maskRank(m)] = uint16_t(j * 64 + lane);
-					r0 += uint32_t(__builtin_popcountll(m));
-				}
-				waveSync();
-				const ViewC V = loadView(views, g0 + v);
-				uint32_t* out = reinterpret_cast<uint32_t*>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
-				for (uint32_t r0 = 0; r0 < run; r0 += 64) {
-					const uint32_t n = min(64u, run - r0);
-					if (lane < n) {
-						const uint32_t e = wave * WAVE_ITEMS + sh.list[wave][r0 + lane];
-						const float4 s = sh.sph[e];
-						const cadrMesh& mesh = meshes[sh.mesh[e]];
-						const uint32_t lod = lodFromDepth(s.w, viewCoord(V.r2, s), V, mesh.lodCount);
-						stageCommand(sh.stage[wave], lane, mesh, lod, T.start + e);
+					for (int j = 0; j < ITEMS; j++) {
+						const uint64_t m = sh.mask[v][wave][j];
+						if ((m >> lane) & 1ull) sh.list[wave][r0 + maskRank(m)] = uint16_t(j * 64 + lane);
+						r0 += uint32_t(__builtin_popcountll(m));
 					}
 					waveSync();
-					flushCommands(out, sh.stage[wave], T.segBase + wavePos + r0, n, J.commandStride, lane);
-					waveSync();
+					const ViewC V = loadView(views, g0 + v);
+					const gmut<uint32_t> out = globalMut<uint32_t>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
+					for (uint32_t r0 = 0; r0 < run; r0 += 64) {
+						const uint32_t n = min(64u, run - r0);
+						if (lane < n) {
+							const uint32_t e = wave * WAVE_ITEMS + sh.list[wave][r0 + lane];
+							const float4 s = sh.sph[e];
+							uint32_t count, first, vofs;
+							meshCommand<decltype(lds)::value>(meshes, sh.mesh[e], s.w, viewCoord(V.r2, s), V, count, first,
+							                                  vofs);
+							stageCommand(sh.stage[wave], lane, count, first, vofs, T.start + e);
+						}
+						waveSync();
+						flushCommands(out, sh.stage[wave], T.segBase + wavePos + r0, n, J.commandStride, lane);
+						waveSync();
+					}
 				}
-			}
+			};
+			if (meshes.inLds)
+				writeViews(std::true_type{});
+			else
+				writeViews(std::false_type{});
 			__syncthreads();
 		}
 	}
