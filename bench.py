@@ -62,6 +62,7 @@ def parse():
     p.add_argument("--served-budget-gib", type=float, default=64.0, help="output memory budget of the server")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-tune", action="store_true", help="use --block/--items instead of the autotuner's choice")
+    p.add_argument("--no-plain", action="store_true", help="skip the per-view plain pass of the batch legs")
     p.add_argument("--scale", type=float, default=1.0, help="root-count scale of full10m (1.0 = 10M drawables)")
     return p.parse_args()
 
@@ -125,65 +126,92 @@ def main():
     batch1 = cull.CullBatch(config, [(ds, views1, out1)])
 
     lib = N.load()
-    level_descs = []
-    prev = None
-    for d, (cnt, lb, pb, wb) in enumerate(ds.levels):
-        lv = np.zeros(1, dtype=N.LEVEL_DTYPE)
-        lv["localMatrices"], lv["worldMatrices"], lv["count"] = lb.address, wb.address, cnt
-        if prev is not None:
-            lv["parentIndex"], lv["parentWorld"], lv["parentCount"] = pb.address, prev[1].address, prev[0]
-        if d == len(ds.levels) - 1:
-            lv["meshIndex"], lv["meshes"], lv["numMeshes"], lv["spheres"] = (ds.mesh_index.address, ds.meshes.address,
-                                                                           ds.num_meshes, ds.spheres.address)
-        level_descs.append(lv)
-        prev = (cnt, wb)
+    level_descs = ds.level_descs(spheres=True)
+    leaf_fused = level_descs[-1].copy()
+    leaf_fused["spheres"] = 0  # the one-view frame needs no world spheres in HBM
+    from cadr_amd.device import Event
+
+    def propagate_inner():
+        for lv in level_descs[:-1]:
+            N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
+
+    # kernel shape of the fused leaf + cull launch: the fastest of the four
+    fused_config, fused_table = cull.make_config(256, args.view_items), None
+    if not args.no_tune:
+        propagate_inner()
+        fused_table = {}
+        for items in (2, 4, 8, 16):
+            cfg = cull.make_config(256, items)
+            b = cull.CullBatch(cfg, [(ds, views1, out1)])
+            b.run_fused(leaf_fused, s)
+            e0, e1 = Event(), Event()
+            e0.record(s)
+            for _ in range(5):
+                b.run_fused(leaf_fused, s)
+            e1.record(s)
+            stream.synchronize()
+            fused_table[f"256x{items}"] = e0.elapsed_ms(e1) / 5
+            b.close()
+        best = min(fused_table, key=fused_table.get)
+        fused_config = cull.make_config(256, int(best.split("x")[1]))
+        log(f"fused autotune: {best}  {fused_table}")
+    batch_fused = cull.CullBatch(fused_config, [(ds, views1, out1)])
 
     total = args.warmup + args.steps
-    t_leaf = Timer(total)
-    t_cull = Timer(total)
 
-    def frame():
-        for d, lv in enumerate(level_descs):
-            last = d == len(level_descs) - 1
-            if last:
-                t_leaf.begin(s)
-            N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
-            if last:
-                t_leaf.end(s)
-        t_cull.begin(s)
-        batch1.run_view(s)
-        t_cull.end(s)
+    def run_frames(fused):
+        """warmup + steps frames; returns (ms per frame, mean ms of the leaf
+        launch, mean ms of the separate cull launch or None, counts)"""
+        t_leaf, t_cull = Timer(total), Timer(total)
 
-    for _ in range(args.warmup):
-        frame()
-    stream.synchronize()
-    # the timed region: exactly `steps` frames, synchronised on both sides
-    t_leaf.starts, t_leaf.ends = t_leaf.starts[args.warmup:], t_leaf.ends[args.warmup:]
-    t_cull.starts, t_cull.ends = t_cull.starts[args.warmup:], t_cull.ends[args.warmup:]
-    t_leaf.i = t_cull.i = 0
-    device.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        frame()
-    stream.synchronize()
-    device.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    _, counts1 = out1.download()
+        def frame():
+            propagate_inner()
+            t_leaf.begin(s)
+            if fused:
+                batch_fused.run_fused(leaf_fused, s)
+            else:
+                N.check(lib.cadrPropagateTransforms(N.ptr(level_descs[-1]), s))
+            t_leaf.end(s)
+            if not fused:
+                t_cull.begin(s)
+                batch1.run_view(s)
+                t_cull.end(s)
+
+        for _ in range(args.warmup):
+            frame()
+        stream.synchronize()
+        # the timed region: exactly `steps` frames, synchronised on both sides
+        for t in (t_leaf, t_cull):
+            t.starts, t.ends, t.i = t.starts[args.warmup:], t.ends[args.warmup:], 0
+        device.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            frame()
+        stream.synchronize()
+        device.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        return ms, t_leaf.mean_ms(), (None if fused else t_cull.mean_ms()), out1.download()[1].copy()
+
+    un_ms, leaf_ms, cull_ms, counts_un = run_frames(False)
+    ms_per_step, fused_ms, _, counts1 = run_frames(True)
     survivors1 = int(counts1.sum())
-    leaf_ms = t_leaf.mean_ms()
-    cull_ms = t_cull.mean_ms()
+    same = bool(np.array_equal(counts1, counts_un))
 
-    # algorithmic bytes of the leaf propagation launch (the dominant kernel):
-    # per leaf 64 B local + 4 B parent index + 4 B mesh index read, 64 B world
-    # + 16 B sphere written; plus each parent's world matrix read once.
     leaves = ds.levels[-1][0]
     parents = ds.levels[-2][0] if len(ds.levels) > 1 else 0
+    inner_nodes = sum(l[0] for l in ds.levels[:-1])
+    # algorithmic bytes of the fused leaf launch (the frame's dominant kernel):
+    # per leaf 64 B local + 4 B parent index + 4 B mesh index read and 64 B
+    # world written, each parent's world matrix read once, 20 B per visible
+    # drawable and 4 B per StateSet written
+    fused_bytes = leaves * (64 + 4 + 4 + 64) + parents * 64 + survivors1 * 20 + ds.K * 4
+    fused_gbs = fused_bytes / (fused_ms * 1e-3) / 1e9
+    # the separate launches: leaf propagation also writes 16-B spheres; the cull
+    # reads 16-B spheres + 4-B mesh indexes
     leaf_bytes = leaves * (64 + 4 + 4 + 64 + 16) + parents * 64
     leaf_gbs = leaf_bytes / (leaf_ms * 1e-3) / 1e9
-    inner_nodes = sum(l[0] for l in ds.levels[:-1])
-    frame_bytes = leaf_bytes + inner_nodes * (64 + 4 + 64) + n * 20 + survivors1 * 20 + ds.K * 4
     cull_bytes = n * 20 + survivors1 * 20 + ds.K * 4
+    frame_bytes = fused_bytes + inner_nodes * (64 + 4 + 64)
     result = {
         "metric": "culled drawables/s + % of HBM roofline; p50/p99 view latency (served)",
         "value": n / (ms_per_step * 1e-3),
@@ -200,21 +228,30 @@ def main():
         "config": {"workload": "full10m: 1000x100x100 transform tree, 10M drawables, 64 StateSets, 3 LODs, "
                                "1 view (propagation + cull + LOD + per-StateSet compaction)",
                    "drawables": n, "nodes": int(sum(l[0] for l in ds.levels)), "statesets": ds.K, "views": 1,
-                   "block": config.blockSize, "items_per_thread": config.itemsPerThread, "autotune": tuned1},
-        "roofline": {"bound": "hbm", "kernel": "propagateKernel<false,true> (leaf level + world spheres)",
-                     "achieved": leaf_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": leaf_gbs / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": leaf_bytes, "avg_ms": leaf_ms},
+                   "frame": "2 propagation launches (root, inner level) + 1 fused leaf propagation + cull launch",
+                   "block": fused_config.blockSize, "items_per_thread": fused_config.itemsPerThread,
+                   "autotune": {"fused_ms": fused_table, "single_view": tuned1}},
+        "roofline": {"bound": "hbm", "kernel": f"cullViewKernel<{fused_config.itemsPerThread},fused> "
+                                               "(leaf propagation + cull + LOD + compaction)",
+                     "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fused_gbs / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": fused_bytes, "avg_ms": fused_ms},
         "frame": {"algorithmic_bytes": frame_bytes, "hbm_frac": frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                  "cull_ms": cull_ms, "cull_gbs": cull_bytes / (cull_ms * 1e-3) / 1e9, "visible": survivors1},
+                  "visible": survivors1, "counts_identical_to_unfused": same},
+        "frame_unfused": {"ms_per_frame": un_ms, "culled_drawables_per_s": n / (un_ms * 1e-3),
+                          "leaf_ms": leaf_ms, "leaf_gbs": leaf_gbs, "leaf_frac": leaf_gbs / HBM_PEAK_GBS,
+                          "leaf_algorithmic_bytes": leaf_bytes, "cull_ms": cull_ms,
+                          "cull_gbs": cull_bytes / (cull_ms * 1e-3) / 1e9,
+                          "cull_config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread}},
     }
-    log(f"full10m: {ms_per_step:.3f} ms/frame, leaf {leaf_ms:.3f} ms ({leaf_gbs:.0f} GB/s), "
-        f"cull {cull_ms:.3f} ms, visible {survivors1}")
+    log(f"full10m fused: {ms_per_step:.3f} ms/frame, fused kernel {fused_ms:.3f} ms ({fused_gbs:.0f} GB/s); "
+        f"unfused {un_ms:.3f} ms/frame (leaf {leaf_ms:.3f} ms {leaf_gbs:.0f} GB/s, cull {cull_ms:.3f} ms); "
+        f"visible {survivors1}, identical={same}")
 
     served_config = cull.make_config(args.block, args.items)
     if not args.no_batch:
         result["batched"], served_config = bench_batched(args, ds, config, stream, lib, N, cull, scenes, device)
     if not args.no_uneven or not args.no_served:
-        batch1.close(); out1.close(); ds.close()
+        batch1.close(); batch_fused.close(); out1.close(); ds.close()
     if not args.no_uneven:
         result["uneven"] = bench_uneven(args, config, stream, lib, N, cull, scenes, device)
     if not args.no_served:
@@ -260,16 +297,11 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
         for j, dj in plain_jobs:
             N.check(lib.cadrCullView(ctypes.byref(view_config), dj.address, N.ptr(j), ws.handle, s))
 
-    for _ in range(2):
-        run_plain()
-    stream.synchronize()
-    t0 = time.perf_counter()
-    reps = max(1, args.batch_steps // 2)
-    for _ in range(reps):
-        run_plain()
-    stream.synchronize()
-    plain_ms = 1e3 * (time.perf_counter() - t0) / reps
-    _, counts_plain = out.download()
+    from cadr_amd.device import Event
+    plain_ms, plain_mean_ms, counts_plain = None, None, None
+    if not args.no_plain:
+        plain_ms, plain_mean_ms = time_plain(run_plain, stream, max(2, args.batch_steps // 2))
+        _, counts_plain = out.download()
 
     # batched, captured once into a HIP graph and replayed
     graph = Graph()
@@ -279,7 +311,6 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
     for _ in range(2):
         graph.launch(s)
     stream.synchronize()
-    from cadr_amd.device import Event
     e0, e1 = Event(), Event()
     t0 = time.perf_counter()
     e0.record(s)
@@ -290,7 +321,7 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
     batch_ms = 1e3 * (time.perf_counter() - t0) / args.batch_steps
     batch_kernel_ms = e0.elapsed_ms(e1) / args.batch_steps
     _, counts = out.download()
-    same = bool(np.array_equal(counts, counts_plain))
+    same = None if counts_plain is None else bool(np.array_equal(counts, counts_plain))
     survivors = int(counts.sum())
     tests = V * n
     alg_bytes = n * 20 + survivors * 20 + V * ds.K * 4
@@ -308,33 +339,19 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
                    "autotune": tuned},
         "plain_config": {"block": view_config.blockSize, "items_per_thread": view_config.itemsPerThread},
         "plain_ms_per_frame": plain_ms,
-        "plain_culled_drawables_per_s": tests / (plain_ms * 1e-3),
-        "speedup_vs_plain": plain_ms / batch_ms,
+        "plain_ms_per_frame_mean": plain_mean_ms,
+        "plain_timing": "HIP events around the 256 launches of a frame, fastest of the repetitions",
+        "plain_culled_drawables_per_s": None if plain_ms is None else tests / (plain_ms * 1e-3),
+        "speedup_vs_plain": None if plain_ms is None else plain_ms / batch_ms,
         "plain_and_batched_counts_identical": same,
     }
     log(f"batch{V}: {batch_ms:.3f} ms/frame ({tests / (batch_ms * 1e-3) / 1e9:.1f} G tests/s, {gbs:.0f} GB/s alg), "
-        f"plain {plain_ms:.3f} ms, speedup {plain_ms / batch_ms:.2f}x, identical={same}")
+        f"plain {plain_ms} ms, identical={same}")
     graph.close()
     batch.close(); out.close(); ws.close()
     for _, dj in plain_jobs:
         dj.close()
     return res, config
-
-
-def level_descs(N, ds):
-    """cadrTransformLevel of every level of a DeviceScene (the last one writes spheres)."""
-    out, prev = [], None
-    for d, (cnt, lb, pb, wb) in enumerate(ds.levels):
-        lv = np.zeros(1, dtype=N.LEVEL_DTYPE)
-        lv["localMatrices"], lv["worldMatrices"], lv["count"] = lb.address, wb.address, cnt
-        if prev is not None:
-            lv["parentIndex"], lv["parentWorld"], lv["parentCount"] = pb.address, prev[1].address, prev[0]
-        if d == len(ds.levels) - 1:
-            lv["meshIndex"], lv["meshes"], lv["numMeshes"], lv["spheres"] = (ds.mesh_index.address, ds.meshes.address,
-                                                                           ds.num_meshes, ds.spheres.address)
-        out.append(lv)
-        prev = (cnt, wb)
-    return out
 
 
 def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
@@ -356,7 +373,7 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     device.synchronize()
     log(f"uneven64: {len(specs)} scenes, {sum(d.n for d in dss)} drawables, "
         f"{sum(v.shape[0] for v in views_l)} views generated in {time.time() - t0:.1f}s")
-    levels = [level_descs(N, ds) for ds in dss]
+    levels = [ds.level_descs(spheres=True) for ds in dss]
     for lvs in levels:
         for lv in lvs:
             N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
@@ -389,17 +406,11 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
             N.check(lib.cadrCullView(ctypes.byref(view_config), dj.address, N.ptr(hj), ws.handle, s))
 
     propagate_all()
-    run_plain()
-    stream.synchronize()
-    reps = 3
     e0, e1 = Event(), Event()
-    e0.record(s)
-    for _ in range(reps):
-        run_plain()
-    e1.record(s)
-    stream.synchronize()
-    plain_ms = e0.elapsed_ms(e1) / reps
-    counts_plain = [e[2].download()[1] for e in entries]
+    plain_ms, plain_mean_ms, counts_plain = None, None, None
+    if not args.no_plain:
+        plain_ms, plain_mean_ms = time_plain(run_plain, stream, 3)
+        counts_plain = [e[2].download()[1] for e in entries]
 
     frame = Graph()
     frame.begin(s)
@@ -428,7 +439,7 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     stream.synchronize()
     cull_ms = e0.elapsed_ms(e1) / steps
     counts = [e[2].download()[1] for e in entries]
-    same = all(np.array_equal(a, b) for a, b in zip(counts, counts_plain))
+    same = None if counts_plain is None else all(np.array_equal(a, b) for a, b in zip(counts, counts_plain))
     survivors = int(sum(c.sum() for c in counts))
     tests = sum(ds.n * v.shape[0] for ds, v in zip(dss, views_l))
     nodes = sum(sum(l[0] for l in ds.levels) for ds in dss)
@@ -451,11 +462,12 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
         "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags,
                    "autotune": tuned},
         "plain_cull_ms": plain_ms,
-        "speedup_cull_vs_plain": plain_ms / cull_ms,
+        "plain_cull_ms_mean": plain_mean_ms,
+        "speedup_cull_vs_plain": None if plain_ms is None else plain_ms / cull_ms,
         "plain_and_batched_counts_identical": same,
     }
     log(f"uneven64: frame {frame_ms:.3f} ms, cull {cull_ms:.3f} ms ({cull_gbs:.0f} GB/s alg), "
-        f"plain cull {plain_ms:.3f} ms ({plain_ms / cull_ms:.1f}x), identical={same}")
+        f"plain cull {plain_ms} ms, identical={same}")
     frame.close(); cull_only.close()
     for _, dj in plain:
         dj.close()
@@ -513,6 +525,24 @@ def copy_calibration(nbytes, stream, lib, N, device):
     stream.synchronize()
     log(f"copy calibration: 3 x {nbytes} bytes device to device")
     src.close(); dst.close()
+
+
+def time_plain(run_plain, stream, reps):
+    """The plain pass (one launch per view) timed with HIP events on its stream:
+    one warm-up frame, then `reps` frames; returns (fastest, mean) ms."""
+    from cadr_amd.device import Event
+    s = stream.handle
+    run_plain()
+    stream.synchronize()
+    times = []
+    for _ in range(reps):
+        e0, e1 = Event(), Event()
+        e0.record(s)
+        run_plain()
+        e1.record(s)
+        stream.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    return min(times), sum(times) / len(times)
 
 
 def tune_key(t):

@@ -33,6 +33,8 @@
 //     pairs skip the per-drawable test when the tile is wholly outside or
 //     wholly inside the frustum.
 #include "internal.h"
+#include "fp32.h"
+#include "matrix_io.h"
 
 #include <algorithm>
 #include <cstddef>
@@ -52,6 +54,17 @@ constexpr uint32_t kStateAggregate = 1;
 constexpr uint32_t kStatePrefix = 2;
 constexpr uint32_t kValueMask = 0x3fffffffu;
 
+// The leaf transform level of a fused propagation + one-view cull launch
+// (cadrPropagateCullView): node i of the level is drawable i of the job.
+struct LeafParams {
+	uint64_t local;        // float mat4[numDrawables]
+	uint64_t parentIndex;  // uint32[numDrawables]; unused for a root level
+	uint64_t parentWorld;  // float mat4[parentCount], 0 for a root level
+	uint64_t world;        // float mat4[numDrawables], written
+	uint64_t spheres;      // float4[numDrawables], written if not 0
+	uint32_t parentCount;
+};
+
 struct CullParams {
 	const cadrCullJob* jobs;
 	const uint32_t* firstWork;   // [numJobs+1] prefix of numTiles
@@ -61,6 +74,7 @@ struct CullParams {
 	uint32_t numJobs;
 	uint32_t totalWork;
 	uint32_t flags;
+	LeafParams leaf;
 };
 
 // Addresses read from descriptors would be generic (FLAT) pointers, and a
@@ -213,26 +227,37 @@ __device__ __forceinline__ uint32_t lodFromDepth(float r, float vz, const ViewC&
 // Tile-vs-view class: 0 = test each drawable, 1 = all outside, 2 = all inside.
 //
 // ts is a sphere containing every drawable sphere of the tile
-// (|c_i - C| + r_i <= R).  Each quantity the drawable test compares,
-// v.z and g(p) = |v.x|*sideX[0] - v.z*sideX[1] (the larger of two plane
-// distances), is Lipschitz in the sphere centre with a constant bounded by
-// L below, so a drawable's quantity differs from the tile centre's by at most
-// L*|c_i - C| <= L*(R - r_i).  With L >= 1 the tile is wholly outside a plane
-// when the centre is outside by more than L*R, and wholly inside when inside
-// by more than L*R.  The margin m, about 2^-14 of the magnitudes involved, is
-// some thousand times the FP32 rounding of the per-drawable expressions, so
-// the float test of every drawable agrees with the class.
+// (|c_i - C| + r_i <= R).  Each quantity the drawable test compares, v.z and
+// g(c) = |v.x|*sideX[0] - v.z*sideX[1] (the larger of two linear plane
+// functions), is Lipschitz in the sphere centre with constant L_P = the
+// largest gradient norm among its linear pieces: |row2| for v.z, and
+// |sideX[0]*row0 -+ sideX[1]*row2| for g (likewise for y), taken at least 1.
+// A drawable's quantity then differs from the tile centre's by at most
+// L*|c_i - C| <= L*(R - r_i), so the tile is wholly outside a plane when the
+// centre is outside by more than L*R, and wholly inside when inside by more
+// than L*R.  The norms are rounded up by a relative 2^-12; the margin m,
+// about 2^-14 of the magnitudes involved, is some thousand times the FP32
+// rounding of the per-drawable expressions, so the float test of every
+// drawable agrees with the class.
+// Bounds only: the hardware square root (within 1 ulp) is covered by the
+// relative slack the callers add.
+__device__ __forceinline__ float norm3(float x, float y, float z)
+{
+	return __builtin_amdgcn_sqrtf((x * x + y * y) + z * z);
+}
+
 __device__ __forceinline__ uint32_t classifyTile(float4 ts, const ViewC& v)
 {
 	const float vx = viewCoord(v.r0, ts);
 	const float vy = viewCoord(v.r1, ts);
 	const float vz = viewCoord(v.r2, ts);
-	const float l0 = sqrtf((v.r0.x * v.r0.x + v.r0.y * v.r0.y) + v.r0.z * v.r0.z);
-	const float l1 = sqrtf((v.r1.x * v.r1.x + v.r1.y * v.r1.y) + v.r1.z * v.r1.z);
-	const float l2 = sqrtf((v.r2.x * v.r2.x + v.r2.y * v.r2.y) + v.r2.z * v.r2.z);
-	float L = fmaxf(1.0f, l2);
-	L = fmaxf(L, l0 * fabsf(v.sx0) + l2 * fabsf(v.sx1));
-	L = fmaxf(L, l1 * fabsf(v.sy0) + l2 * fabsf(v.sy1));
+	const float4 a = v.r0, b = v.r1, c = v.r2;
+	float L = fmaxf(1.0f, norm3(c.x, c.y, c.z));
+	L = fmaxf(L, norm3(v.sx0 * a.x - v.sx1 * c.x, v.sx0 * a.y - v.sx1 * c.y, v.sx0 * a.z - v.sx1 * c.z));
+	L = fmaxf(L, norm3(v.sx0 * a.x + v.sx1 * c.x, v.sx0 * a.y + v.sx1 * c.y, v.sx0 * a.z + v.sx1 * c.z));
+	L = fmaxf(L, norm3(v.sy0 * b.x - v.sy1 * c.x, v.sy0 * b.y - v.sy1 * c.y, v.sy0 * b.z - v.sy1 * c.z));
+	L = fmaxf(L, norm3(v.sy0 * b.x + v.sy1 * c.x, v.sy0 * b.y + v.sy1 * c.y, v.sy0 * b.z + v.sy1 * c.z));
+	L *= 1.0f + 1.0f / 4096.0f;
 	const float R = ts.w * L * (1.0f + 1.0f / 4096.0f);
 	const float mag = fabsf(ts.x) + fabsf(ts.y) + fabsf(ts.z) + ts.w + fabsf(v.r0.w) + fabsf(v.r1.w) +
 	                  fabsf(v.r2.w) + fabsf(v.nearZ) + fabsf(v.farZ) + 1.0f;
@@ -363,6 +388,18 @@ __device__ __forceinline__ void meshCommand(const MeshTable& M, uint32_t m, floa
 	}
 }
 
+// Local bounding sphere {centre, radius} of mesh m (m < numMeshes).
+template <bool LDS>
+__device__ __forceinline__ float4 meshSphere(const MeshTable& M, uint32_t m)
+{
+	if constexpr (LDS) {
+		const cadrMesh& mesh = M.lds[m];
+		return make_float4(mesh.center[0], mesh.center[1], mesh.center[2], mesh.radius);
+	} else {
+		return loadF4(reinterpret_cast<gptr<float4>>(M.mem + m));
+	}
+}
+
 // A wave's compacted commands are staged in LDS (5 dwords each, a stride
 // that is conflict-free across the LDS banks) and then stored as runs of 64
 // consecutive dwords, so every store instruction writes 256 contiguous bytes
@@ -384,6 +421,12 @@ __device__ __forceinline__ void flushCommands(gmut<uint32_t> out, const uint32_t
                                               uint32_t stride, uint32_t lane)
 {
 	const gmut<uint32_t> dst = out + uint64_t(first) * 5ull;
+	if (n == 64u && first + 64u <= stride) {
+		// a full run inside the stride: no per-dword bounds, constant offsets
+#pragma unroll
+		for (uint32_t k = 0; k < 5; k++) __builtin_nontemporal_store(stage[k * 64u + lane], dst + (k * 64u + lane));
+		return;
+	}
 #pragma unroll
 	for (uint32_t k = 0; k < 5; k++) {
 		const uint32_t d = k * 64u + lane;
@@ -403,12 +446,84 @@ __device__ __forceinline__ bool stageMeshes(cadrMesh* sMeshes, const cadrCullJob
 	return true;
 }
 
+// The fused form of loadTileItems: the wave's items are computed from the
+// leaf level instead of read, exactly as cadrPropagateTransforms computes
+// them (world = parentWorld[parentIndex[i]] * local[i], a parent index at or
+// past parentCount or a root level giving world = local; the world sphere of
+// mesh meshIndex[i], or the empty sphere for an index outside the table).
+// World matrices (and spheres, if requested) are stored on the way.  Item j
+// of the lane is tile offset base + 64*j + lane; the wave's 64 matrices of
+// item j are contiguous, so they move through LDS as one 4-KiB block.
+// Parent and mesh indices of all items are loaded first (clamped into the
+// tile, like loadTileItems), so each item then needs one round trip for its
+// local and parent matrices.
+template <int ITEMS, bool LDS>
+__device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const cadrCullJob& J, const TileDesc& T,
+                                                   uint32_t base, uint32_t lane, float4* stage,
+                                                   const MeshTable& meshes, float4 (&sp)[ITEMS],
+                                                   uint32_t (&mi)[ITEMS])
+{
+	const float4 empty = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
+	if (T.count == 0) {
+#pragma unroll
+		for (int j = 0; j < ITEMS; j++) {
+			sp[j] = empty;
+			mi[j] = 0;
+		}
+		return;
+	}
+	const bool root = Lf.parentWorld == 0;
+	const gptr<uint32_t> parentIndex = globalPtr<uint32_t>(Lf.parentIndex);
+	const gptr<uint32_t> meshIndex = globalPtr<uint32_t>(J.meshIndex);
+	const uint32_t last = T.count - 1;
+	uint32_t pi[ITEMS];
+#pragma unroll
+	for (int j = 0; j < ITEMS; j++) {
+		const uint32_t node = T.start + min(base + uint32_t(j) * 64u + lane, last);
+		pi[j] = root ? ~0u : parentIndex[node];
+		mi[j] = meshIndex[node];
+	}
+	const uint64_t end = uint64_t(T.start) + T.count;
+	const gptr<float4> local = globalPtr<float4>(Lf.local);
+	const gptr<float4> parentWorld = globalPtr<float4>(Lf.parentWorld);
+	const gmut<float4> world = globalMut<float4>(Lf.world);
+#pragma unroll
+	for (int j = 0; j < ITEMS; j++) {
+		const uint32_t off = base + uint32_t(j) * 64u;  // tile offset of lane 0 (wave-uniform)
+		if (off >= T.count) {
+			sp[j] = empty;
+			continue;
+		}
+		const uint64_t first = uint64_t(T.start) + off;
+		const uint64_t i = first + lane;
+		const bool live = i < end;
+		float4 L[4], W[4];
+		loadMatrices(local, first, end, stage, lane, L);
+		const uint32_t p = pi[j];
+		if (live && p < Lf.parentCount) {
+			float4 P[4];
+#pragma unroll
+			for (int c = 0; c < 4; c++) P[c] = loadF4(parentWorld + (4 * uint64_t(p) + c));
+			mat4Mul(P, L, W);
+		} else {
+#pragma unroll
+			for (int c = 0; c < 4; c++) W[c] = L[c];
+		}
+		storeMatrices(world, first, end, stage, lane, W);
+		float4 s = empty;
+		if (live && mi[j] < J.numMeshes) s = sphereTransform(W, meshSphere<LDS>(meshes, mi[j]));
+		if (Lf.spheres && live) storeF4(globalMut<float4>(Lf.spheres) + i, s);
+		sp[j] = s;
+	}
+}
+
 // ---------------------------------------------------------------- one view
 
-template <int ITEMS, bool LDS>
+template <int ITEMS, bool LDS, bool FUSED>
 __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCullJob& J, uint32_t t, uint32_t epoch,
                                              const MeshTable& meshes, uint32_t (*sCnt)[ITEMS], uint32_t* sExclP,
-                                             uint32_t* sStage, uint32_t tid, uint32_t lane, uint32_t wave)
+                                             uint32_t* sStage, float4* sMat, uint32_t tid, uint32_t lane,
+                                             uint32_t wave)
 {
 	constexpr int BLOCK = 256, WAVES = 4, WAVE_ITEMS = 64 * ITEMS;
 	const TileDesc T = loadTile(J, t, BLOCK * ITEMS);
@@ -417,10 +532,14 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 	const gptr<uint32_t> meshIndex = globalPtr<uint32_t>(J.meshIndex);
 
 	// all loads first (indices clamped into the tile, so none is conditional),
-	// then the tests: a wave keeps ITEMS 16-byte loads in flight at once
+	// then the tests: a wave keeps ITEMS 16-byte loads in flight at once; the
+	// fused form computes the spheres from the leaf level instead
 	float4 sp[ITEMS];
 	uint32_t mi[ITEMS];
-	loadTileItems<ITEMS>(spheres, meshIndex, T, wave * WAVE_ITEMS + lane, sp, mi);
+	if constexpr (FUSED)
+		propagateTileItems<ITEMS, LDS>(P.leaf, J, T, wave * WAVE_ITEMS, lane, sMat, meshes, sp, mi);
+	else
+		loadTileItems<ITEMS>(spheres, meshIndex, T, wave * WAVE_ITEMS + lane, sp, mi);
 	float vz[ITEMS];
 	uint64_t mask[ITEMS];
 #pragma unroll
@@ -476,10 +595,11 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 }
 
 
-template <int ITEMS>
+template <int ITEMS, bool FUSED>
 __global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
 {
 	constexpr int BLOCK = 256, WAVES = 4;
+	__shared__ float4 sMat[FUSED ? WAVES : 1][FUSED ? 256 : 1];
 	__shared__ uint32_t sTicket;
 	__shared__ uint32_t sCnt[WAVES][ITEMS];
 	__shared__ uint32_t sExcl;
@@ -495,10 +615,11 @@ __global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
 	const uint32_t t = uniform(sTicket);
 	const MeshTable meshes{sMeshes, globalPtr<cadrMesh>(J.meshes), ldsMeshes};
 	if (t < P.totalWork) {
+		float4* mat = &sMat[FUSED ? wave : 0][0];
 		if (ldsMeshes)
-			cullViewTile<ITEMS, true>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, tid, lane, wave);
+			cullViewTile<ITEMS, true, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, mat, tid, lane, wave);
 		else
-			cullViewTile<ITEMS, false>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, tid, lane, wave);
+			cullViewTile<ITEMS, false, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, mat, tid, lane, wave);
 	}
 
 	// the last workgroup to finish resets the ticket and starts a new epoch
@@ -525,10 +646,13 @@ struct CullShared {
 	uint64_t mask[kViewGroup][WAVES][ITEMS];
 	uint32_t cnt[kViewGroup][WAVES];
 	uint32_t excl[kViewGroup];
+	uint32_t cls[kViewGroup];
+	uint64_t active;
 	uint16_t list[WAVES][64 * ITEMS];
 	uint32_t stage[WAVES][320];
 	cadrMesh meshes[kLdsMeshes];
 	float red[WAVES][6];
+	float red2[WAVES];
 	uint32_t bad[WAVES];
 	uint32_t ticket;
 };
@@ -589,6 +713,8 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 				m = 0;
 			}
 			sp[j] = s;
+			sp[j] = s;
+			mi[j] = m;
 			sh.sph[e] = s;
 			sh.mesh[e] = m;
 			// a sphere other than a finite one with r >= 0 or the empty sphere
@@ -633,9 +759,25 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 					mx[a] = fmaxf(mx[a], sh.red[q][3 + a]);
 				}
 			}
-			const float hx = (mx[0] - mn[0]) * 0.5f, hy = (mx[1] - mn[1]) * 0.5f, hz = (mx[2] - mn[2]) * 0.5f;
+			// centre: the middle of the tile's bounding box; radius: the largest
+			// |c_i - C| + r_i over the tile (a second reduction), which for
+			// scattered drawables is up to sqrt(3) times tighter than the box's
+			// half diagonal
 			const float cx = (mn[0] + mx[0]) * 0.5f, cy = (mn[1] + mx[1]) * 0.5f, cz = (mn[2] + mx[2]) * 0.5f;
-			const float R = sqrtf(hx * hx + hy * hy + hz * hz);
+			float rad = 0.f;
+#pragma unroll
+			for (int j = 0; j < ITEMS; j++) {
+				const float4 q = sp[j];
+				if (q.w >= 0.f) rad = fmaxf(rad, norm3(q.x - cx, q.y - cy, q.z - cz) + q.w);
+			}
+#pragma unroll
+			for (int off = 32; off >= 1; off >>= 1) rad = fmaxf(rad, __shfl_xor(rad, off));
+			if (lane == 0) sh.red2[wave] = rad;
+			__syncthreads();
+			float R = 0.f;
+#pragma unroll
+			for (int q = 0; q < WAVES; q++) R = fmaxf(R, sh.red2[q]);
+			R *= 1.0f + 1.0f / 4096.0f;
 			tileUsable = anyBad == 0u && __builtin_isfinite(R) && __builtin_isfinite(cx) &&
 			             __builtin_isfinite(cy) && __builtin_isfinite(cz);
 			// inflate against rounding of the centre and radius above
@@ -651,18 +793,25 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		for (uint32_t g0 = 0; g0 < J.numViews; g0 += kViewGroup) {
 			const uint32_t vg = uniform(min(kViewGroup, J.numViews - g0));
 
-			// tile classes of the group's views, lane v holding view g0+v
-			uint32_t cls = 0;
-			if (emptyTile) cls = 1;
-			else if (tileUsable && lane < vg) cls = classifyTile(tileSphere, loadView(views, g0 + lane));
+			// tile classes of the group's views (wave 0, lane v holding view
+			// g0+v), and the views that see part of the tile; the others
+			// contribute nothing and are skipped by all three phases
+			if (wave == 0) {
+				uint32_t c = 0;
+				if (emptyTile) c = 1;
+				else if (tileUsable && lane < vg) c = classifyTile(tileSphere, loadView(views, g0 + lane));
+				const uint64_t act = __ballot(lane < vg && c != 1u);
+				sh.cls[lane] = c;
+				if (lane == 0) sh.active = act;
+			}
+			__syncthreads();
+			const uint64_t activeLds = sh.active;
+			const uint64_t active = (uint64_t(uniform(uint32_t(activeLds >> 32))) << 32) | uniform(uint32_t(activeLds));
 
 			// 1. visibility bits and per-wave counts
-			for (uint32_t v = 0; v < vg; v++) {
-				const uint32_t c = uniform(__builtin_amdgcn_readlane(cls, v));
-				if (c == 1u) {
-					if (lane == 0) sh.cnt[v][wave] = 0;
-					continue;
-				}
+			for (uint64_t am = active; am != 0ull; am &= am - 1ull) {
+				const uint32_t v = uint32_t(__builtin_ctzll(am));
+				const uint32_t c = uniform(sh.cls[v]);
 				const ViewC V = loadView(views, g0 + v);
 				uint32_t n = 0;
 #pragma unroll
@@ -680,8 +829,10 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			// 2. look-back, one view per lane of wave 0
 			if (wave == 0 && lane < vg) {
 				uint32_t total = 0;
+				if ((active >> lane) & 1ull) {
 #pragma unroll
-				for (int q = 0; q < WAVES; q++) total += sh.cnt[lane][q];
+					for (int q = 0; q < WAVES; q++) total += sh.cnt[lane][q];
+				}
 				uint64_t* chain = chain0 + uint64_t(g0 + lane) * J.numTiles;
 				uint32_t excl = 0;
 				if (T.first) {
@@ -699,12 +850,40 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 
 			// 3. compacted writes: lane r of the wave stores its r-th visible drawable
 			auto writeViews = [&](auto lds) {
-				for (uint32_t v = 0; v < vg; v++) {
+				for (uint64_t am = active; am != 0ull; am &= am - 1ull) {
+					const uint32_t v = uint32_t(__builtin_ctzll(am));
 					const uint32_t run = uniform(sh.cnt[v][wave]);
 					if (run == 0) continue;
 					uint32_t wavePos = sh.excl[v];
 					for (uint32_t q = 0; q < wave; q++) wavePos += sh.cnt[v][q];
 					wavePos = uniform(wavePos);
+					const ViewC V = loadView(views, g0 + v);
+					const gmut<uint32_t> out = globalMut<uint32_t>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
+					if (2u * run >= uint32_t(WAVE_ITEMS)) {
+						// dense: each item's visible drawables straight from registers,
+						// one flush per item (mostly whole 64-command runs)
+						uint32_t pos = T.segBase + wavePos;
+#pragma unroll
+						for (int j = 0; j < ITEMS; j++) {
+							const uint64_t m = sh.mask[v][wave][j];
+							const uint32_t n = uint32_t(__builtin_popcountll(m));
+							if (n == 0) continue;
+							if ((m >> lane) & 1ull) {
+								uint32_t count, first, vofs;
+								meshCommand<decltype(lds)::value>(meshes, mi[j], sp[j].w, viewCoord(V.r2, sp[j]), V,
+								                                  count, first, vofs);
+								stageCommand(sh.stage[wave], maskRank(m), count, first, vofs,
+								             T.start + wave * WAVE_ITEMS + j * 64 + lane);
+							}
+							waveSync();
+							flushCommands(out, sh.stage[wave], pos, n, J.commandStride, lane);
+							waveSync();
+							pos += n;
+						}
+						continue;
+					}
+					// sparse: gather the visible drawables of all items into a list,
+					// then write them 64 at a time
 					uint32_t r0 = 0;
 #pragma unroll
 					for (int j = 0; j < ITEMS; j++) {
@@ -713,8 +892,6 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 						r0 += uint32_t(__builtin_popcountll(m));
 					}
 					waveSync();
-					const ViewC V = loadView(views, g0 + v);
-					const gmut<uint32_t> out = globalMut<uint32_t>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
 					for (uint32_t r0 = 0; r0 < run; r0 += 64) {
 						const uint32_t n = min(64u, run - r0);
 						if (lane < n) {
@@ -822,13 +999,18 @@ struct KernelEntry {
 	uint32_t block, items;
 	KernelFn batch;   // many views / jobs
 	KernelFn view;    // one view (256-thread blocks only), or nullptr
+	KernelFn fused;   // leaf propagation + one view, or nullptr
 };
 
 const KernelEntry kKernels[] = {
-    {256, 2, cullKernel<256, 2>, cullViewKernel<2>},   {256, 4, cullKernel<256, 4>, cullViewKernel<4>},
-    {256, 8, cullKernel<256, 8>, cullViewKernel<8>},   {256, 16, cullKernel<256, 16>, cullViewKernel<16>},
-    {512, 2, cullKernel<512, 2>, nullptr},             {512, 4, cullKernel<512, 4>, nullptr},
-    {128, 8, cullKernel<128, 8>, nullptr},             {128, 16, cullKernel<128, 16>, nullptr},
+    {256, 2, cullKernel<256, 2>, cullViewKernel<2, false>, cullViewKernel<2, true>},
+    {256, 4, cullKernel<256, 4>, cullViewKernel<4, false>, cullViewKernel<4, true>},
+    {256, 8, cullKernel<256, 8>, cullViewKernel<8, false>, cullViewKernel<8, true>},
+    {256, 16, cullKernel<256, 16>, cullViewKernel<16, false>, cullViewKernel<16, true>},
+    {512, 2, cullKernel<512, 2>, nullptr, nullptr},
+    {512, 4, cullKernel<512, 4>, nullptr, nullptr},
+    {128, 8, cullKernel<128, 8>, nullptr, nullptr},
+    {128, 16, cullKernel<128, 16>, nullptr, nullptr},
 };
 
 const KernelEntry* findKernel(const cadrCullConfig* c)
@@ -1016,5 +1198,42 @@ CADR_API int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, cons
 	p.flags = config->flags;
 	hipLaunchKernelGGL(ke->view, dim3(hostJob->numTiles), dim3(256), 0, hipStream_t(stream), p);
 	CADR_CHECK_HIP(hipGetLastError(), "cullViewKernel launch");
+	return ok();
+}
+
+CADR_API int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCullConfig* config, uint64_t deviceJob,
+                                   const cadrCullJob* hostJob, cadrCullWorkspace* ws, cadrStream stream)
+{
+	CADR_REQUIRE(leaf && hostJob, "cadrPropagateCullView: null argument");
+	CADR_REQUIRE(hostJob->numViews == 1, "cadrPropagateCullView: the job must hold exactly one view");
+	const KernelEntry* ke = findKernel(config);
+	CADR_REQUIRE(ke && ke->fused, "cadrPropagateCullView: the config needs blockSize 256 and itemsPerThread 2/4/8/16");
+	CADR_REQUIRE(ws, "cadrPropagateCullView: workspace is NULL");
+	CADR_REQUIRE(deviceJob && deviceJob % 16 == 0, "cadrPropagateCullView: deviceJob must be a 16-byte aligned address");
+	CADR_REQUIRE(leaf->count == hostJob->numDrawables, "cadrPropagateCullView: the leaf level must hold the job's drawables");
+	CADR_REQUIRE(leaf->localMatrices && leaf->worldMatrices, "cadrPropagateCullView: null matrix array");
+	CADR_REQUIRE((leaf->localMatrices % 16) == 0 && (leaf->worldMatrices % 16) == 0 && (leaf->parentWorld % 16) == 0 &&
+	                 (leaf->spheres % 16) == 0,
+	             "cadrPropagateCullView: matrix and sphere arrays must be 16-byte aligned");
+	CADR_REQUIRE(leaf->parentWorld == 0 || (leaf->parentIndex && leaf->parentIndex % 4 == 0),
+	             "cadrPropagateCullView: parentIndex is NULL or misaligned for a non-root level");
+	const int rc = checkJob(*hostJob);
+	if (rc != CADR_SUCCESS) return rc;
+	CADR_REQUIRE(hostJob->numTiles <= ws->statusWords, "cadrPropagateCullView: workspace has too few status words");
+	int dev = 0;
+	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
+	CADR_REQUIRE(dev == ws->device, "cadrPropagateCullView: workspace belongs to another device");
+	if (hostJob->numTiles == 0) return ok();
+	CullParams p{};
+	p.jobs = reinterpret_cast<const cadrCullJob*>(deviceJob);
+	p.status = ws->status;
+	p.ctl = ws->ctl;
+	p.numJobs = 1;
+	p.totalWork = hostJob->numTiles;
+	p.flags = config->flags;
+	p.leaf = LeafParams{leaf->localMatrices, leaf->parentIndex, leaf->parentWorld, leaf->worldMatrices, leaf->spheres,
+	                    leaf->parentWorld ? leaf->parentCount : 0u};
+	hipLaunchKernelGGL(ke->fused, dim3(hostJob->numTiles), dim3(256), 0, hipStream_t(stream), p);
+	CADR_CHECK_HIP(hipGetLastError(), "fused cullViewKernel launch");
 	return ok();
 }

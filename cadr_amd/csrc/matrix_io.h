@@ -1,0 +1,69 @@
+// Wave-wide movement of column-major float mat4 arrays between HBM and
+// registers, shared by the propagation kernels and the fused
+// propagation + cull kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cadr {
+
+// 16-byte element access through any pointer type (generic or an explicit
+// address space), component by component so that address-space-qualified
+// pointers need no float4 copy constructor; the loads and stores merge into
+// one 16-byte instruction.
+template <class P>
+__device__ __forceinline__ float4 loadF4(P p)
+{
+	return make_float4(p->x, p->y, p->z, p->w);
+}
+
+template <class P>
+__device__ __forceinline__ void storeF4(P p, float4 v)
+{
+	p->x = v.x;
+	p->y = v.y;
+	p->z = v.z;
+	p->w = v.w;
+}
+
+// A wave's 64 matrices (4 KiB, contiguous in memory) move between HBM and
+// registers through LDS: four 16-byte loads per lane at lane stride 16 B (each
+// instruction one contiguous KiB), then each lane reads its own 64 bytes from
+// LDS; stores go the same way in reverse.  Matrix `first` is lane 0's; lanes
+// whose matrix index is at or past `count` neither load nor store.
+template <class Src>
+__device__ __forceinline__ void loadMatrices(Src src, uint64_t first, uint64_t count, float4* stage, uint32_t lane,
+                                             float4 M[4])
+{
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		const uint32_t q = uint32_t(k) * 64u + lane;      // float4 index inside the wave's 4 KiB
+		if (first + q / 4u < count) stage[q] = loadF4(src + (4 * first + q));
+	}
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+	for (int c = 0; c < 4; c++) M[c] = stage[4 * lane + c];
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class Dst>
+__device__ __forceinline__ void storeMatrices(Dst dst, uint64_t first, uint64_t count, float4* stage, uint32_t lane,
+                                              const float4 M[4])
+{
+#pragma unroll
+	for (int c = 0; c < 4; c++) stage[4 * lane + c] = M[c];
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		const uint32_t q = uint32_t(k) * 64u + lane;
+		if (first + q / 4u < count) storeF4(dst + (4 * first + q), stage[q]);
+	}
+}
+
+}  // namespace cadr

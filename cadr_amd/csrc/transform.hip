@@ -17,6 +17,7 @@
 // instruction moves one contiguous KiB.
 #include "internal.h"
 #include "fp32.h"
+#include "matrix_io.h"
 
 namespace cadr {
 
@@ -29,44 +30,6 @@ __device__ __forceinline__ float4 nodeSphere(const float4 W[4], const cadrMesh* 
 	if (m >= numMeshes) return make_float4(0.f, 0.f, 0.f, -__builtin_inff());
 	const float4 local = *reinterpret_cast<const float4*>(meshes + m);  // center.xyz, radius
 	return sphereTransform(W, local);
-}
-
-// A wave's 64 matrices (4 KiB, contiguous in memory) move between HBM and
-// registers through LDS: four 16-byte loads per lane at lane stride 16 B (each
-// instruction one contiguous KiB), then each lane reads its own 64 bytes from
-// LDS; stores go the same way in reverse.  Lanes past the end of the level
-// neither load nor store.
-__device__ __forceinline__ void loadMatrices(const float4* __restrict__ src, uint64_t first, uint64_t count,
-                                             float4* stage, uint32_t lane, float4 M[4])
-{
-#pragma unroll
-	for (int k = 0; k < 4; k++) {
-		const uint32_t q = uint32_t(k) * 64u + lane;      // float4 index inside the wave's 4 KiB
-		if (first + q / 4u < count) stage[q] = src[4 * first + q];
-	}
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-	for (int c = 0; c < 4; c++) M[c] = stage[4 * lane + c];
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ void storeMatrices(float4* __restrict__ dst, uint64_t first, uint64_t count,
-                                              float4* stage, uint32_t lane, const float4 M[4])
-{
-#pragma unroll
-	for (int c = 0; c < 4; c++) stage[4 * lane + c] = M[c];
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-	for (int k = 0; k < 4; k++) {
-		const uint32_t q = uint32_t(k) * 64u + lane;
-		if (first + q / 4u < count) dst[4 * first + q] = stage[q];
-	}
 }
 
 template <bool ROOT, bool SPHERES>

@@ -144,6 +144,23 @@ class DeviceScene:
             N.check(lib.cadrPropagateTransforms(N.ptr(lv), stream))
             prev = (cnt, wb)
 
+    def level_descs(self, spheres: bool = True) -> list:
+        """cadrTransformLevel of every level, root first; the last one writes
+        the world spheres when `spheres`."""
+        out, prev = [], None
+        for d, (cnt, lb, pb, wb) in enumerate(self.levels):
+            lv = np.zeros(1, dtype=N.LEVEL_DTYPE)
+            lv["localMatrices"], lv["worldMatrices"], lv["count"] = lb.address, wb.address, cnt
+            if prev is not None and pb is not None:
+                lv["parentIndex"], lv["parentWorld"], lv["parentCount"] = pb.address, prev[1].address, prev[0]
+            if spheres and d == len(self.levels) - 1:
+                lv["meshIndex"], lv["meshes"], lv["numMeshes"] = (self.mesh_index.address, self.meshes.address,
+                                                                  self.num_meshes)
+                lv["spheres"] = self.spheres.address
+            out.append(lv)
+            prev = (cnt, wb)
+        return out
+
     def world_matrices(self, level: int = -1) -> np.ndarray:
         cnt, _, _, wb = self.levels[level]
         return wb.download(np.float32, cnt * 16).reshape(cnt, 16)
@@ -236,6 +253,13 @@ class CullBatch:
         assert self.host_jobs.shape[0] == 1 and self.host_jobs["numViews"][0] == 1
         N.check(N.load().cadrCullView(ctypes.byref(self.config), self.device_jobs.address, N.ptr(self.host_jobs),
                                       self.workspace.handle, stream))
+
+    def run_fused(self, leaf: np.ndarray, stream=None) -> None:
+        """cadrPropagateCullView: the leaf level (LEVEL_DTYPE, 1 element) fused
+        with the one-view cull of this batch's single job."""
+        assert self.host_jobs.shape[0] == 1 and self.host_jobs["numViews"][0] == 1
+        N.check(N.load().cadrPropagateCullView(N.ptr(leaf), ctypes.byref(self.config), self.device_jobs.address,
+                                               N.ptr(self.host_jobs), self.workspace.handle, stream))
 
     def close(self):
         self.device_jobs.close()

@@ -343,6 +343,17 @@ int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, const cadrC
 int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCullJob* hostJob,
                  cadrCullWorkspace* workspace, cadrStream stream);
 
+/* The last transform level fused with the one-view cull, in one launch:
+ * cadrPropagateTransforms(leaf) followed by cadrCullView(job), computed
+ * tile by tile so that the world spheres never travel through HBM (they are
+ * still written when leaf->spheres != 0).  Node i of the leaf level is
+ * drawable i of the job (leaf->count == numDrawables); the spheres are made
+ * from the job's meshIndex and meshes (the leaf's own meshIndex/meshes are
+ * not read).  World matrices, spheres, commands and counts are bit-identical
+ * to the two separate calls.  config: blockSize 256, itemsPerThread 2/4/8/16. */
+int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCullConfig* config, uint64_t deviceJob,
+                          const cadrCullJob* hostJob, cadrCullWorkspace* workspace, cadrStream stream);
+
 /* Kernel tile size for a config (blockSize * itemsPerThread), 0 if unsupported. */
 uint32_t cadrCullTileSize(const cadrCullConfig* config);
 

@@ -229,3 +229,73 @@ def test_cull_view_after_batch_shares_workspace(plain):
     for o in outs + [multi_out]:
         o.close()
     ws.close()
+
+
+@pytest.mark.parametrize("items", [2, 4, 8, 16])
+def test_fused_propagation_and_cull(gpu, items):
+    """cadrPropagateCullView against cadrPropagateTransforms + cadrCullView and the
+    oracle: world matrices, spheres, commands and counts bit for bit; tree and
+    flat (root) leaf levels, a mesh table too large for LDS, spheres on and off."""
+    from cadr_amd.device import DeviceBuffer, synchronize
+    cases = [
+        ("tree", scenes.tree_scene((40, 50, 60), seed=scenes.SEED_BASE + 31, num_statesets=64, lod_count=3,
+                                   trange=60.0)),
+        ("flat", scenes.plain_scene(70_001, seed=scenes.SEED_BASE + 32, num_statesets=7)),
+        ("100 meshes", scenes.tree_scene((300, 130), seed=scenes.SEED_BASE + 33, num_statesets=5, lod_count=2,
+                                         num_meshes=100, trange=200.0)),
+    ]
+    cam = scenes.Camera(eye=(10.0, -20.0, 150.0), dir=(-0.1, 0.1, -1.0), up=(0.0, 1.0, 0.0), fovy=1.1)
+    views = cull.make_views([cam])
+    config = cull.make_config(256, items)
+    for name, sc in cases:
+        print(f"fused {name} items={items}", flush=True)
+        worlds, sph = oracle.propagate_scene(sc)
+        exp_cmds, exp_counts = oracle.cull(sph, sc.mesh_index, sc.meshes, sc.stateset_counts, views,
+                                           cmd_dtype=N.CMD_DTYPE)
+        assert exp_counts.sum() > 100
+        ds = cull.DeviceScene(sc)
+        descs = ds.level_descs(spheres=True)
+        for lv in descs[:-1]:
+            N.check(N.load().cadrPropagateTransforms(N.ptr(lv), None))
+        for with_spheres in (True, False):
+            out = cull.CullOutputs(1, ds.n, ds.K)
+            ds.spheres.upload(np.full((ds.n, 4), np.nan, np.float32))
+            ds.levels[-1][3].upload(np.full((ds.n, 16), np.nan, np.float32))
+            leaf = descs[-1].copy()
+            if not with_spheres:
+                leaf["spheres"] = 0
+            b = cull.CullBatch(config, [(ds, views, out)])
+            b.run_fused(leaf)
+            synchronize()
+            cmds, counts = out.download()
+            _compare(cmds, counts, exp_cmds, exp_counts, sc.stateset_counts, f"fused {name}")
+            assert ds.world_matrices(-1).tobytes() == worlds[-1].tobytes(), f"{name}: world matrices"
+            got_sph = ds.download_spheres()
+            if with_spheres:
+                assert got_sph.tobytes() == sph.tobytes(), f"{name}: spheres"
+            else:
+                assert np.isnan(got_sph).all(), f"{name}: spheres written although not requested"
+            b.close(); out.close()
+        ds.close()
+
+
+def test_fused_rejects_bad_arguments(gpu):
+    sc = scenes.plain_scene(1000, seed=scenes.SEED_BASE + 34)
+    ds = cull.DeviceScene(sc)
+    views = cull.make_views([scenes.DEFAULT_CAMERA])
+    out = cull.CullOutputs(1, ds.n, ds.K)
+    leaf = ds.level_descs()[-1].copy()
+    for cfg in (cull.make_config(512, 4), cull.make_config(256, 3)):
+        b = cull.CullBatch(cull.make_config(256, 4), [(ds, views, out)])
+        b.config = cfg
+        with pytest.raises(N.CadrError):
+            b.run_fused(leaf)
+        b.close()
+    b = cull.CullBatch(cull.make_config(256, 4), [(ds, views, out)])
+    bad = leaf.copy(); bad["count"] = 999
+    with pytest.raises(N.CadrError):
+        b.run_fused(bad)
+    bad = leaf.copy(); bad["worldMatrices"] = 0
+    with pytest.raises(N.CadrError):
+        b.run_fused(bad)
+    b.close(); out.close(); ds.close()
