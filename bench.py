@@ -483,11 +483,23 @@ def bench_served(args, config, N, cull, scenes, device):
     batched across scenes by the serving loop; latency from scheduled arrival
     to the result on the host."""
     from cadr_amd.serve import Server
+    from cadr_amd.tune import autotune, describe
     t0 = time.time()
     sizes = scenes.served_specs()
     dss = [cull.DeviceScene(scenes.served_synth(i, n)) for i, n in enumerate(sizes)]
     device.synchronize()
     views = cull.make_views(scenes.batch_cameras(256))
+    tuned = None
+    if not args.no_tune:
+        # a launch as the server forms it at moderate load: the 16 hot scenes,
+        # two views each
+        outs = [cull.CullOutputs(2, ds.n, ds.K) for ds in dss[:16]]
+        entries = [(ds, views[2 * i:2 * i + 2], out) for i, (ds, out) in enumerate(zip(dss[:16], outs))]
+        config, table = autotune(entries, reps=3)
+        tuned = {"choice": describe(config), "candidates_ms": {tune_key(t): t["ms"] for t in table}}
+        log(f"served128 autotune (16 scenes x 2 views): {describe(config)}  {tuned['candidates_ms']}")
+        for out in outs:
+            out.close()
     srv = Server(block=config.blockSize, items=config.itemsPerThread, flags=config.flags, max_batch_views=256, inflight=4,
                  budget_bytes=int(args.served_budget_gib * (1 << 30)))
     for ds in dss:
@@ -510,7 +522,8 @@ def bench_served(args, config, N, cull, scenes, device):
                         "Poisson arrivals, 80% of requests to the 16 hottest scenes, views as batch256",
             "drawables_resident": int(sum(sizes)), "output_budget_gib": args.served_budget_gib,
             "max_batch_views": 256, "inflight_batches": 4,
-            "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags},
+            "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags,
+                       "autotune": tuned},
             "runs": runs}
 
 

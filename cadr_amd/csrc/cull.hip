@@ -452,11 +452,37 @@ __device__ __forceinline__ bool stageMeshes(cadrMesh* sMeshes, const cadrCullJob
 // past parentCount or a root level giving world = local; the world sphere of
 // mesh meshIndex[i], or the empty sphere for an index outside the table).
 // World matrices (and spheres, if requested) are stored on the way.  Item j
-// of the lane is tile offset base + 64*j + lane; the wave's 64 matrices of
-// item j are contiguous, so they move through LDS as one 4-KiB block.
-// Parent and mesh indices of all items are loaded first (clamped into the
-// tile, like loadTileItems), so each item then needs one round trip for its
-// local and parent matrices.
+// of the lane is tile offset base + 64*j + lane; the wave's 64 local
+// matrices of item j are one contiguous 4-KiB block, loaded lane-contiguous
+// (each instruction one KiB) and redistributed through LDS.  Parent and mesh
+// indices of all items are loaded first (clamped into the tile, like
+// loadTileItems); the local block and parent matrix of item j+1 are issued
+// before item j is computed, so each item's loads overlap the previous
+// item's work.
+struct ItemLoads {
+	float4 raw[4];  // this lane's 4 float4 of the wave's 4-KiB local block
+	float4 P[4];    // this lane's parent world matrix
+};
+
+__device__ __forceinline__ void issueItemLoads(const LeafParams& Lf, uint64_t first, uint64_t end, uint32_t p,
+                                               bool live, uint32_t lane, ItemLoads& d)
+{
+	const gptr<float4> local = globalPtr<float4>(Lf.local);
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		const uint32_t q = uint32_t(k) * 64u + lane;  // float4 index inside the block
+		d.raw[k] = first + q / 4u < end ? loadF4(local + (4 * first + q)) : make_float4(0.f, 0.f, 0.f, 0.f);
+	}
+	if (live && p < Lf.parentCount) {
+		const gptr<float4> parentWorld = globalPtr<float4>(Lf.parentWorld);
+#pragma unroll
+		for (int c = 0; c < 4; c++) d.P[c] = loadF4(parentWorld + (4 * uint64_t(p) + c));
+	} else {
+#pragma unroll
+		for (int c = 0; c < 4; c++) d.P[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+	}
+}
+
 template <int ITEMS, bool LDS>
 __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const cadrCullJob& J, const TileDesc& T,
                                                    uint32_t base, uint32_t lane, float4* stage,
@@ -484,27 +510,38 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 		mi[j] = meshIndex[node];
 	}
 	const uint64_t end = uint64_t(T.start) + T.count;
-	const gptr<float4> local = globalPtr<float4>(Lf.local);
-	const gptr<float4> parentWorld = globalPtr<float4>(Lf.parentWorld);
 	const gmut<float4> world = globalMut<float4>(Lf.world);
+	// items whose 64 nodes all lie past the tile end come last and are empty
+	const uint32_t items = base >= T.count ? 0u : min(uint32_t(ITEMS), (T.count - base + 63u) / 64u);
+	ItemLoads next;
+	if (items > 0) issueItemLoads(Lf, uint64_t(T.start) + base, end, pi[0], uint64_t(T.start) + base + lane < end,
+	                              lane, next);
 #pragma unroll
 	for (int j = 0; j < ITEMS; j++) {
-		const uint32_t off = base + uint32_t(j) * 64u;  // tile offset of lane 0 (wave-uniform)
-		if (off >= T.count) {
+		if (uint32_t(j) >= items) {
 			sp[j] = empty;
 			continue;
 		}
-		const uint64_t first = uint64_t(T.start) + off;
+		const uint64_t first = uint64_t(T.start) + base + uint32_t(j) * 64u;
 		const uint64_t i = first + lane;
 		const bool live = i < end;
-		float4 L[4], W[4];
-		loadMatrices(local, first, end, stage, lane, L);
-		const uint32_t p = pi[j];
-		if (live && p < Lf.parentCount) {
-			float4 P[4];
+		const ItemLoads cur = next;
+		if (uint32_t(j) + 1u < items)
+			issueItemLoads(Lf, first + 64u, end, pi[j + 1 < ITEMS ? j + 1 : j], first + 64u + lane < end, lane, next);
+		// redistribute the local block: lane l takes matrix l
 #pragma unroll
-			for (int c = 0; c < 4; c++) P[c] = loadF4(parentWorld + (4 * uint64_t(p) + c));
-			mat4Mul(P, L, W);
+		for (int k = 0; k < 4; k++) stage[uint32_t(k) * 64u + lane] = cur.raw[k];
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		float4 L[4], W[4];
+#pragma unroll
+		for (int c = 0; c < 4; c++) L[c] = stage[4 * lane + c];
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		if (live && pi[j] < Lf.parentCount) {
+			mat4Mul(cur.P, L, W);
 		} else {
 #pragma unroll
 			for (int c = 0; c < 4; c++) W[c] = L[c];
@@ -646,8 +683,6 @@ struct CullShared {
 	uint64_t mask[kViewGroup][WAVES][ITEMS];
 	uint32_t cnt[kViewGroup][WAVES];
 	uint32_t excl[kViewGroup];
-	uint32_t cls[kViewGroup];
-	uint64_t active;
 	uint16_t list[WAVES][64 * ITEMS];
 	uint32_t stage[WAVES][320];
 	cadrMesh meshes[kLdsMeshes];
@@ -793,25 +828,19 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		for (uint32_t g0 = 0; g0 < J.numViews; g0 += kViewGroup) {
 			const uint32_t vg = uniform(min(kViewGroup, J.numViews - g0));
 
-			// tile classes of the group's views (wave 0, lane v holding view
-			// g0+v), and the views that see part of the tile; the others
+			// tile classes of the group's views, lane v holding view g0+v, and
+			// the views that see part of the tile (every wave computes the same
+			// classes, so the mask is the same in every wave); the others
 			// contribute nothing and are skipped by all three phases
-			if (wave == 0) {
-				uint32_t c = 0;
-				if (emptyTile) c = 1;
-				else if (tileUsable && lane < vg) c = classifyTile(tileSphere, loadView(views, g0 + lane));
-				const uint64_t act = __ballot(lane < vg && c != 1u);
-				sh.cls[lane] = c;
-				if (lane == 0) sh.active = act;
-			}
-			__syncthreads();
-			const uint64_t activeLds = sh.active;
-			const uint64_t active = (uint64_t(uniform(uint32_t(activeLds >> 32))) << 32) | uniform(uint32_t(activeLds));
+			uint32_t cls = 0;
+			if (emptyTile) cls = 1;
+			else if (tileUsable && lane < vg) cls = classifyTile(tileSphere, loadView(views, g0 + lane));
+			const uint64_t active = __ballot(lane < vg && cls != 1u);
 
 			// 1. visibility bits and per-wave counts
 			for (uint64_t am = active; am != 0ull; am &= am - 1ull) {
 				const uint32_t v = uint32_t(__builtin_ctzll(am));
-				const uint32_t c = uniform(sh.cls[v]);
+				const uint32_t c = uniform(__builtin_amdgcn_readlane(cls, v));
 				const ViewC V = loadView(views, g0 + v);
 				uint32_t n = 0;
 #pragma unroll
@@ -859,9 +888,9 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 					wavePos = uniform(wavePos);
 					const ViewC V = loadView(views, g0 + v);
 					const gmut<uint32_t> out = globalMut<uint32_t>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
-					if (2u * run >= uint32_t(WAVE_ITEMS)) {
-						// dense: each item's visible drawables straight from registers,
-						// one flush per item (mostly whole 64-command runs)
+					if (run == uint32_t(WAVE_ITEMS)) {
+						// every drawable of the wave visible: each item straight from
+						// registers, one whole 64-command run per item
 						uint32_t pos = T.segBase + wavePos;
 #pragma unroll
 						for (int j = 0; j < ITEMS; j++) {
@@ -882,8 +911,8 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 						}
 						continue;
 					}
-					// sparse: gather the visible drawables of all items into a list,
-					// then write them 64 at a time
+					// otherwise gather the visible drawables of all items into a
+					// list, then write them 64 at a time
 					uint32_t r0 = 0;
 #pragma unroll
 					for (int j = 0; j < ITEMS; j++) {
