@@ -57,6 +57,7 @@ def parse():
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--no-uneven", action="store_true")
     p.add_argument("--no-served", action="store_true")
+    p.add_argument("--no-gather", action="store_true")
     p.add_argument("--served-rates", default="5000,10000,20000,50000", help="Poisson arrival rates (req/s)")
     p.add_argument("--served-seconds", type=float, default=1.0, help="arrival window per rate")
     p.add_argument("--served-budget-gib", type=float, default=64.0, help="output memory budget of the server")
@@ -250,8 +251,10 @@ def main():
     served_config = cull.make_config(args.block, args.items)
     if not args.no_batch:
         result["batched"], served_config = bench_batched(args, ds, config, stream, lib, N, cull, scenes, device)
-    if not args.no_uneven or not args.no_served:
+    if not args.no_uneven or not args.no_served or not args.no_gather:
         batch1.close(); batch_fused.close(); out1.close(); ds.close()
+    if not args.no_gather:
+        result["gather"] = bench_gather(args, stream, lib, N, scenes, device)
     if not args.no_uneven:
         result["uneven"] = bench_uneven(args, config, stream, lib, N, cull, scenes, device)
     if not args.no_served:
@@ -281,12 +284,14 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
         tuned = {"choice": describe(config), "candidates_ms": {tune_key(t): t["ms"] for t in table}}
         log(f"batch{V} autotune: {describe(config)}  {tuned['candidates_ms']}")
     batch = cull.CullBatch(config, [(ds, views, out)])
-    # plain: one single-view launch per view
-    ws = cull.Workspace(max(batch.status_words, 1), 4)
-    # per-view jobs that write straight into the batched output arrays (same layout)
+    # plain: one single-view launch per view, with tiles built for its own
+    # tile size, writing straight into the batched output arrays (same layout)
+    vtiles, vnt = ds.tiles(cull.tile_size(view_config))
+    ws = cull.Workspace(max(vnt, 1), 4)
     plain_jobs = []
     for v in range(V):
         j = batch.host_jobs.copy()
+        j["tiles"], j["numTiles"] = vtiles.address, vnt
         j["views"] = batch.host_jobs["views"][0] + v * N.VIEW_DTYPE.itemsize
         j["commands"] = out.commands.address + v * n * 20
         j["counts"] = out.counts.address + v * ds.K * 4
@@ -389,12 +394,16 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
             for lv in lvs:
                 N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
 
-    # plain: one single-view launch per (scene, view), same outputs
-    ws = cull.Workspace(max(batch.status_words, 1), 4)
+    # plain: one single-view launch per (scene, view), same outputs, tiles
+    # built for the single-view tile size
+    vtile = cull.tile_size(view_config)
+    ws = cull.Workspace(max(max(ds.tiles(vtile)[1] for ds in dss), 1), 4)
     plain = []
     for j, (ds, views, out) in enumerate(entries):
+        vtiles, vnt = ds.tiles(vtile)
         for v in range(views.shape[0]):
             hj = batch.host_jobs[j:j + 1].copy()
+            hj["tiles"], hj["numTiles"] = vtiles.address, vnt
             hj["views"] = batch.host_jobs["views"][j] + v * N.VIEW_DTYPE.itemsize
             hj["commands"] = out.commands.address + v * ds.n * 20
             hj["counts"] = out.counts.address + v * ds.K * 4
@@ -538,6 +547,76 @@ def copy_calibration(nbytes, stream, lib, N, device):
     stream.synchronize()
     log(f"copy calibration: 3 x {nbytes} bytes device to device")
     src.close(); dst.close()
+
+
+def bench_gather(args, stream, lib, N, scenes, device, n=10_000_000):
+    """The reference's own pass (processDrawables.comp, handle level 3) at
+    10M drawables: 48-B DrawableGpuData records, 3-level handle lookups,
+    PrimitiveSet and MatrixList-header reads, 16-B indirect + 32-B pointer
+    writes.  Outputs checked against the closed form; the oracle (one
+    thread) timed on a 1M-drawable level-3 sample."""
+    from cadr_amd.device import DeviceBuffer, Event
+    s = stream.handle
+    t0 = time.time()
+    g = scenes.GatherScene(n)
+    size = g.layout(0)["size"]
+    img_buf = DeviceBuffer(size)
+    img, push, level = g.image(img_buf.address)
+    img_buf.upload(img)
+    ind, ptr = DeviceBuffer(16 * n), DeviceBuffer(32 * n)
+    pc = N.PushConstants(push[0], push[1], ind.address, ptr.address)
+    leaf_bytes = (g.layout(0)["leaf_range"][1] - g.layout(0)["leaf_range"][0] + 1) * 2048 * 8
+    del img
+    log(f"gather: {n} drawables, handle level {level}, image {size / 1e9:.2f} GB built in {time.time() - t0:.1f}s")
+
+    def run():
+        N.check(lib.cadrProcessDrawables(ctypes.byref(pc), level, n, s))
+
+    for _ in range(2):
+        run()
+    stream.synchronize()
+    e0, e1 = Event(), Event()
+    steps = 10
+    e0.record(s)
+    for _ in range(steps):
+        run()
+    e1.record(s)
+    stream.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    ei, ep = g.expected(img_buf.address)
+    same = bool(np.array_equal(ind.download(np.uint32, 4 * n).reshape(n, 4), ei) and
+                np.array_equal(ptr.download(np.uint64, 4 * n).reshape(n, 4), ep))
+    del ei, ep
+    # per drawable: 48-B record + the MatrixList header line (64 B) read, 16 + 32 B
+    # written; the last-level handle tables read once
+    alg = n * (48 + 64 + 16 + 32) + leaf_bytes
+    gbs = alg / (ms * 1e-3) / 1e9
+    img_buf.close(); ind.close(); ptr.close()
+    # CPU: the oracle's gather, one thread, on a 1M-drawable level-3 image
+    import oracle
+    m = 1_000_000
+    gs = scenes.GatherScene(m, first_handle=1 << 22)
+    base, ib, pb = 1 << 44, 1 << 45, 3 << 44
+    simg, spush, slevel = gs.image(base)
+    sind, sptr = np.zeros(4 * m, np.uint32), np.zeros(4 * m, np.uint64)
+    regions = [(base, simg), (ib, sind.view(np.uint8)), (pb, sptr.view(np.uint8))]
+    times = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        oracle.process_drawables(regions, (spush[0], spush[1], ib, pb), slevel, m)
+        times.append(time.perf_counter() - t1)
+    cpu_rate = m / min(times)
+    res = {"workload": f"gather: processDrawables at {n} drawables, handle level {level} (64 geometries, one "
+                       "MatrixList per drawable, 4 PrimitiveSets per geometry)",
+           "ms": ms, "drawables_per_s": n / (ms * 1e-3),
+           "roofline": {"bound": "hbm", "kernel": f"processDrawablesKernel<{level}>", "achieved": gbs,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": alg},
+           "outputs_match_closed_form": same,
+           "cpu_oracle": {"value": cpu_rate, "unit": "drawables/s", "cores": 1,
+                          "sample": f"{m} drawables, handle level 3, oracle/cadr_oracle.c, best of 3"}}
+    log(f"gather: {ms:.3f} ms ({n / (ms * 1e-3) / 1e9:.1f} G drawables/s, {gbs:.0f} GB/s alg), match={same}, "
+        f"cpu oracle {cpu_rate / 1e6:.1f} M drawables/s")
+    return res
 
 
 def time_plain(run_plain, stream, reps):

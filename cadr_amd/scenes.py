@@ -437,3 +437,132 @@ def served_specs(num_scenes: int = 128, lo: float = 1e4, hi: float = 2e7, budget
     if budget_drawables is not None and sizes.sum() > budget_drawables:
         sizes = np.maximum(1, (sizes * (budget_drawables / sizes.sum())).astype(np.int64))
     return [int(s) for s in sizes]
+
+
+# ------------------------------------------------------------ gather input
+
+class GatherScene:
+    """Input of the reference's gather pass (processDrawables.comp) in one
+    device image, built with numpy in bulk: handle tables of the reference's
+    layout (2048 uint64 entries per table, HandleTable.h:33-35), `geometries`
+    Geometry allocations (vertex, index, 4 PrimitiveSet records {count,
+    first}), one MatrixList {numMatrices = 1, capacity = 1, pad} + mat4 per
+    drawable (128 B, MatrixList.h:54-59), one shared drawable-data
+    allocation, and the 48-B DrawableGpuData list (Drawable.h:32-43).
+
+    Handles are numbered consecutively from `first_handle` (geometries
+    first, then MatrixLists, then the drawable-data handle), as the
+    reference's HandleTable numbers allocations; first_handle >= 2^22 puts
+    the scene at handle level 3 whatever its size."""
+
+    TABLE = 2048
+
+    def __init__(self, n: int, seed: int = SEED_BASE + 6, geometries: int = 64, first_handle: int = 1):
+        self.n, self.G = n, geometries
+        self.h0 = first_handle
+        self.hmax = first_handle + 3 * geometries + n  # the drawable-data handle
+        self.level = 1 if self.hmax < 2048 else (2 if self.hmax < (1 << 22) else 3)
+        u = rand_u64(seed, 50, 0, n, 2)
+        self.geom = ((u[:, 0] >> np.uint64(32)) * np.uint64(geometries) >> np.uint64(32)).astype(np.int64)
+        self.ps = ((u[:, 1] >> np.uint64(32)) * np.uint64(4) >> np.uint64(32)).astype(np.int64)
+        pv = rand_u64(seed, 51, 0, geometries, 8)
+        self.ps_count = (pv[:, 0:4] >> np.uint64(47)).astype(np.uint32) + np.uint32(1)
+        self.ps_first = (pv[:, 4:8] >> np.uint64(33)).astype(np.uint32)
+
+    def layout(self, base: int):
+        """Byte offsets of every region of the image at device address base."""
+        T = self.TABLE * 8
+        lo, hi = self.h0 >> 11, self.hmax >> 11
+        L = {"root": 0}
+        off = T
+        if self.level == 3:
+            mlo, mhi = self.h0 >> 22, self.hmax >> 22
+            L["mid"], L["mid_range"] = off, (mlo, mhi)
+            off += (mhi - mlo + 1) * T
+        if self.level >= 2:
+            L["leaf"], L["leaf_range"] = off, (lo, hi)
+            off += (hi - lo + 1) * T
+        L["geom"] = off
+        off += self.G * 192
+        L["dd"] = off
+        off += 64
+        L["ml"] = off
+        off += self.n * 128
+        L["list"] = off
+        off += self.n * 48
+        L["size"] = off
+        self.base = base
+        return L
+
+    def addresses(self, base: int) -> np.ndarray:
+        """Device address of every handle h0..hmax (index h - h0)."""
+        L = self.layout(base)
+        a = np.zeros(self.hmax - self.h0 + 1, np.uint64)
+        g = np.arange(self.G, dtype=np.uint64)
+        gbase = np.uint64(base + L["geom"]) + g * np.uint64(192)
+        a[0:3 * self.G:3] = gbase                    # vertex data
+        a[1:3 * self.G:3] = gbase + np.uint64(64)    # index data
+        a[2:3 * self.G:3] = gbase + np.uint64(128)   # PrimitiveSet records
+        a[3 * self.G:3 * self.G + self.n] = np.uint64(base + L["ml"]) + np.arange(self.n, dtype=np.uint64) * np.uint64(128)
+        a[-1] = base + L["dd"]
+        return a
+
+    def image(self, base: int):
+        """(uint8 image, push constants (table, list, 0, 0), level)."""
+        L = self.layout(base)
+        img = np.zeros(L["size"], np.uint8)
+        addr = self.addresses(base)
+        h = np.arange(self.h0, self.hmax + 1, dtype=np.int64)
+        T = self.TABLE
+        if self.level == 1:
+            img[:T * 8].view(np.uint64)[h] = addr
+        else:
+            lo, hi = L["leaf_range"]
+            leaf = img[L["leaf"]:L["leaf"] + (hi - lo + 1) * T * 8].view(np.uint64)
+            leaf[h - lo * T] = addr
+            leaf_addr = np.uint64(base + L["leaf"]) + np.arange(hi - lo + 1, dtype=np.uint64) * np.uint64(T * 8)
+            if self.level == 2:
+                img[:T * 8].view(np.uint64)[lo:hi + 1] = leaf_addr
+            else:
+                mlo, mhi = L["mid_range"]
+                mid = img[L["mid"]:L["mid"] + (mhi - mlo + 1) * T * 8].view(np.uint64)
+                leaves = np.arange(lo, hi + 1, dtype=np.int64)
+                mid[leaves - mlo * T] = leaf_addr           # mid table (L >> 11), entry L & 2047
+                img[:T * 8].view(np.uint64)[mlo:mhi + 1] = (np.uint64(base + L["mid"]) +
+                                                            np.arange(mhi - mlo + 1, dtype=np.uint64) * np.uint64(T * 8))
+        geo = img[L["geom"]:L["geom"] + self.G * 192].reshape(self.G, 192)
+        ps = np.zeros((self.G, 4, 2), np.uint32)
+        ps[:, :, 0], ps[:, :, 1] = self.ps_count, self.ps_first
+        geo[:, 128:160] = ps.reshape(self.G, 8).view(np.uint8)
+        ml = img[L["ml"]:L["ml"] + self.n * 128].view(np.uint32).reshape(self.n, 32)
+        ml[:, 0] = 1   # numMatrices
+        ml[:, 1] = 1   # capacity
+        ml[:, 16] = 0x3F800000  # the matrix: 1.0 on the diagonal
+        ml[:, 21] = 0x3F800000
+        ml[:, 26] = 0x3F800000
+        ml[:, 31] = 0x3F800000
+        from ._native import DRAWABLE_DTYPE
+        d = img[L["list"]:L["list"] + self.n * 48].view(DRAWABLE_DTYPE)
+        gh = np.uint64(self.h0) + np.uint64(3) * self.geom.astype(np.uint64)
+        d["vertexDataHandle"] = gh
+        d["indexDataHandle"] = gh + np.uint64(1)
+        d["primitiveSetHandle"] = gh + np.uint64(2)
+        d["matrixListHandle"] = np.uint64(self.h0 + 3 * self.G) + np.arange(self.n, dtype=np.uint64)
+        d["drawableDataHandle"] = np.uint64(self.hmax)
+        d["primitiveSetOffset"] = (8 * self.ps).astype(np.uint32)
+        return img, (base, base + L["list"], 0, 0), self.level
+
+    def expected(self, base: int):
+        """Closed-form outputs: indirect uint32[n,4], pointers uint64[n,4]."""
+        L = self.layout(base)
+        ind = np.zeros((self.n, 4), np.uint32)
+        ind[:, 0] = self.ps_count[self.geom, self.ps]
+        ind[:, 1] = 1
+        ind[:, 2] = self.ps_first[self.geom, self.ps]
+        ptr = np.zeros((self.n, 4), np.uint64)
+        gbase = np.uint64(base + L["geom"]) + self.geom.astype(np.uint64) * np.uint64(192)
+        ptr[:, 0] = gbase
+        ptr[:, 1] = gbase + np.uint64(64)
+        ptr[:, 2] = np.uint64(base + L["ml"]) + np.arange(self.n, dtype=np.uint64) * np.uint64(128)
+        ptr[:, 3] = base + L["dd"]
+        return ind, ptr

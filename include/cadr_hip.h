@@ -302,7 +302,9 @@ typedef struct cadrCullJob {
 	uint64_t spheres;           /* float4[numDrawables]                        */
 	uint64_t meshIndex;         /* uint32[numDrawables]                        */
 	uint64_t meshes;            /* cadrMesh[]                                  */
-	uint64_t tiles;             /* cadrCullTile[numTiles] built for the launch tile size */
+	uint64_t tiles;             /* cadrCullTile[numTiles] from cadrBuildCullTiles for the
+	                             * launch's tile size (cadrCullTileSize); a tile longer
+	                             * than that is processed only up to it */
 	uint64_t views;             /* cadrView[numViews]                          */
 	uint64_t commands;          /* cadrDrawIndexedIndirectCommand[numViews][commandStride] */
 	uint64_t counts;            /* uint32[numViews][numStateSets]              */
