@@ -232,7 +232,7 @@ def main():
                    "frame": "2 propagation launches (root, inner level) + 1 fused leaf propagation + cull launch",
                    "block": fused_config.blockSize, "items_per_thread": fused_config.itemsPerThread,
                    "autotune": {"fused_ms": fused_table, "single_view": tuned1}},
-        "roofline": {"bound": "hbm", "kernel": f"cullViewKernel<{fused_config.itemsPerThread},fused> "
+        "roofline": {"bound": "hbm", "kernel": f"cullFusedKernel<{fused_config.itemsPerThread}> "
                                                "(leaf propagation + cull + LOD + compaction)",
                      "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fused_gbs / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes": fused_bytes, "avg_ms": fused_ms},

@@ -633,7 +633,7 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 
 
 template <int ITEMS, bool FUSED>
-__global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
+__device__ __forceinline__ void cullViewBody(const CullParams& P)
 {
 	constexpr int BLOCK = 256, WAVES = 4;
 	__shared__ float4 sMat[FUSED ? WAVES : 1][FUSED ? 256 : 1];
@@ -672,6 +672,21 @@ __global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
 	}
 }
 
+template <int ITEMS>
+__global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
+{
+	cullViewBody<ITEMS, false>(P);
+}
+
+// The fused kernel is latency-bound (a chain of dependent loads per item), so
+// occupancy pays: WPE waves per SIMD is requested from the register allocator
+// (4 for the small item counts, where it costs no spills).
+template <int ITEMS, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void cullFusedKernel(CullParams P)
+{
+	cullViewBody<ITEMS, true>(P);
+}
+
 // ------------------------------------------------------- many views, jobs
 
 template <int BLOCK, int ITEMS>
@@ -684,7 +699,7 @@ struct CullShared {
 	uint32_t cnt[kViewGroup][WAVES];
 	uint32_t excl[kViewGroup];
 	uint16_t list[WAVES][64 * ITEMS];
-	uint32_t stage[WAVES][320];
+	uint32_t stage[WAVES][640];
 	cadrMesh meshes[kLdsMeshes];
 	float red[WAVES][6];
 	float red2[WAVES];
@@ -888,10 +903,13 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 					wavePos = uniform(wavePos);
 					const ViewC V = loadView(views, g0 + v);
 					const gmut<uint32_t> out = globalMut<uint32_t>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
-					if (run == uint32_t(WAVE_ITEMS)) {
-						// every drawable of the wave visible: each item straight from
-						// registers, one whole 64-command run per item
-						uint32_t pos = T.segBase + wavePos;
+					if (4u * run >= uint32_t(WAVE_ITEMS)) {
+						// dense (a quarter or more of the wave visible): each item's
+						// visible drawables go straight from registers into a
+						// 128-command staging ring, stored 64 at a time
+						uint32_t* stage = sh.stage[wave];
+						uint32_t pos = T.segBase + wavePos;  // output slot of stage[0]
+						uint32_t pend = 0;                   // staged, not yet stored
 #pragma unroll
 						for (int j = 0; j < ITEMS; j++) {
 							const uint64_t m = sh.mask[v][wave][j];
@@ -901,13 +919,35 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 								uint32_t count, first, vofs;
 								meshCommand<decltype(lds)::value>(meshes, mi[j], sp[j].w, viewCoord(V.r2, sp[j]), V,
 								                                  count, first, vofs);
-								stageCommand(sh.stage[wave], maskRank(m), count, first, vofs,
+								stageCommand(stage, pend + maskRank(m), count, first, vofs,
 								             T.start + wave * WAVE_ITEMS + j * 64 + lane);
 							}
+							pend += n;
+							if (pend >= 64u) {
+								waveSync();
+								flushCommands(out, stage, pos, 64u, J.commandStride, lane);
+								pos += 64u;
+								pend -= 64u;
+								// the overflow (fewer than 64 commands) moves to the front
+								uint32_t carry[5];
+#pragma unroll
+								for (uint32_t k = 0; k < 5; k++) {
+									const uint32_t d = k * 64u + lane;
+									carry[k] = d < 5u * pend ? stage[320u + d] : 0u;
+								}
+								waveSync();
+#pragma unroll
+								for (uint32_t k = 0; k < 5; k++) {
+									const uint32_t d = k * 64u + lane;
+									if (d < 5u * pend) stage[d] = carry[k];
+								}
+								waveSync();
+							}
+						}
+						if (pend > 0u) {
 							waveSync();
-							flushCommands(out, sh.stage[wave], pos, n, J.commandStride, lane);
+							flushCommands(out, stage, pos, pend, J.commandStride, lane);
 							waveSync();
-							pos += n;
 						}
 						continue;
 					}
@@ -1032,10 +1072,10 @@ struct KernelEntry {
 };
 
 const KernelEntry kKernels[] = {
-    {256, 2, cullKernel<256, 2>, cullViewKernel<2, false>, cullViewKernel<2, true>},
-    {256, 4, cullKernel<256, 4>, cullViewKernel<4, false>, cullViewKernel<4, true>},
-    {256, 8, cullKernel<256, 8>, cullViewKernel<8, false>, cullViewKernel<8, true>},
-    {256, 16, cullKernel<256, 16>, cullViewKernel<16, false>, cullViewKernel<16, true>},
+    {256, 2, cullKernel<256, 2>, cullViewKernel<2>, cullFusedKernel<2, 4>},
+    {256, 4, cullKernel<256, 4>, cullViewKernel<4>, cullFusedKernel<4, 4>},
+    {256, 8, cullKernel<256, 8>, cullViewKernel<8>, cullFusedKernel<8, 1>},
+    {256, 16, cullKernel<256, 16>, cullViewKernel<16>, cullFusedKernel<16, 1>},
     {512, 2, cullKernel<512, 2>, nullptr, nullptr},
     {512, 4, cullKernel<512, 4>, nullptr, nullptr},
     {128, 8, cullKernel<128, 8>, nullptr, nullptr},
