@@ -36,6 +36,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` as measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration) and
+    recorded in a profiles/*.md summary as a line
+    `traffic_bytes_per_launch <kernel> <bytes>`; None if not recorded."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(PROFILES, "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+(\d+)", line)
+                if m and m.group(1) == kernel:
+                    return int(m.group(2))
+    return None
 
 
 def log(*a):
@@ -235,7 +252,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": f"cullFusedKernel<{fused_config.itemsPerThread}> "
                                                "(leaf propagation + cull + LOD + compaction)",
                      "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fused_gbs / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": fused_bytes, "avg_ms": fused_ms},
+                     "traffic": measured_traffic(f"cullFusedKernel<{fused_config.itemsPerThread}>"),
+                     "algorithmic_bytes": fused_bytes, "avg_ms": fused_ms},
         "frame": {"algorithmic_bytes": frame_bytes, "hbm_frac": frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                   "visible": survivors1, "counts_identical_to_unfused": same},
         "frame_unfused": {"ms_per_frame": un_ms, "culled_drawables_per_s": n / (un_ms * 1e-3),

@@ -358,10 +358,16 @@ __device__ uint32_t lookBackWave(const uint64_t* chain, uint32_t c, uint32_t epo
 }
 
 // The mesh table of a job: an LDS copy when it is small, else device memory.
+// The LDS copy is field-major (dword f of mesh m at f*kLdsMeshes + m): with
+// 64-byte records every mesh's copy of a field would sit in the same two LDS
+// banks, and 64 lanes reading random meshes would serialise.
+// cadrMesh dwords: 0-2 centre, 3 radius, 4 lodCount, 5 vertexOffset,
+// 8-11 indexCount[4], 12-15 firstIndex[4].
 struct MeshTable {
-	const cadrMesh* lds;
+	const uint32_t* lds;
 	gptr<cadrMesh> mem;
 	bool inLds;
+	__device__ __forceinline__ uint32_t field(uint32_t f, uint32_t m) const { return lds[f * kLdsMeshes + m]; }
 };
 
 // Command fields of mesh m at the LOD the view selects for a sphere of radius
@@ -374,11 +380,10 @@ __device__ __forceinline__ void meshCommand(const MeshTable& M, uint32_t m, floa
                                             uint32_t& count, uint32_t& first, uint32_t& vofs)
 {
 	if constexpr (LDS) {
-		const cadrMesh& mesh = M.lds[m];
-		const uint32_t lod = lodFromDepth(r, vz, V, mesh.lodCount);
-		count = mesh.indexCount[lod];
-		first = mesh.firstIndex[lod];
-		vofs = uint32_t(mesh.vertexOffset);
+		const uint32_t lod = lodFromDepth(r, vz, V, M.field(4, m));
+		count = M.field(8 + lod, m);
+		first = M.field(12 + lod, m);
+		vofs = M.field(5, m);
 	} else {
 		const gptr<cadrMesh> mesh = M.mem + m;
 		const uint32_t lod = lodFromDepth(r, vz, V, mesh->lodCount);
@@ -393,8 +398,8 @@ template <bool LDS>
 __device__ __forceinline__ float4 meshSphere(const MeshTable& M, uint32_t m)
 {
 	if constexpr (LDS) {
-		const cadrMesh& mesh = M.lds[m];
-		return make_float4(mesh.center[0], mesh.center[1], mesh.center[2], mesh.radius);
+		return make_float4(__uint_as_float(M.field(0, m)), __uint_as_float(M.field(1, m)),
+		                   __uint_as_float(M.field(2, m)), __uint_as_float(M.field(3, m)));
 	} else {
 		return loadF4(reinterpret_cast<gptr<float4>>(M.mem + m));
 	}
@@ -434,15 +439,14 @@ __device__ __forceinline__ void flushCommands(gmut<uint32_t> out, const uint32_t
 	}
 }
 
-// Mesh table of a job staged in LDS when it is small (one 16-byte word per
-// thread); returns whether the LDS copy is in use.
+// Mesh table of a job staged in LDS, field-major (see MeshTable), when it is
+// small; returns whether the LDS copy is in use.
 template <int BLOCK>
-__device__ __forceinline__ bool stageMeshes(cadrMesh* sMeshes, const cadrCullJob& J, uint32_t tid)
+__device__ __forceinline__ bool stageMeshes(uint32_t* sMeshes, const cadrCullJob& J, uint32_t tid)
 {
 	if (J.numMeshes > kLdsMeshes) return false;
 	const gptr<uint32_t> src = globalPtr<uint32_t>(J.meshes);
-	uint32_t* dst = reinterpret_cast<uint32_t*>(sMeshes);
-	for (uint32_t i = tid; i < J.numMeshes * 16u; i += BLOCK) dst[i] = src[i];
+	for (uint32_t i = tid; i < J.numMeshes * 16u; i += BLOCK) sMeshes[(i % 16u) * kLdsMeshes + i / 16u] = src[i];
 	return true;
 }
 
@@ -530,13 +534,13 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 			issueItemLoads(Lf, first + 64u, end, pi[j + 1 < ITEMS ? j + 1 : j], first + 64u + lane < end, lane, next);
 		// redistribute the local block: lane l takes matrix l
 #pragma unroll
-		for (int k = 0; k < 4; k++) stage[uint32_t(k) * 64u + lane] = cur.raw[k];
+		for (int k = 0; k < 4; k++) stage[stageSlot(uint32_t(k) * 64u + lane)] = cur.raw[k];
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 		float4 L[4], W[4];
 #pragma unroll
-		for (int c = 0; c < 4; c++) L[c] = stage[4 * lane + c];
+		for (int c = 0; c < 4; c++) L[c] = stage[stageSlot(4 * lane + c)];
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -640,7 +644,7 @@ __device__ __forceinline__ void cullViewBody(const CullParams& P)
 	__shared__ uint32_t sTicket;
 	__shared__ uint32_t sCnt[WAVES][ITEMS];
 	__shared__ uint32_t sExcl;
-	__shared__ cadrMesh sMeshes[kLdsMeshes];
+	__shared__ uint32_t sMeshes[16 * kLdsMeshes];
 	__shared__ uint32_t sStage[WAVES * 320];
 
 	const uint32_t tid = threadIdx.x, lane = laneId(), wave = tid / 64;
@@ -700,7 +704,7 @@ struct CullShared {
 	uint32_t excl[kViewGroup];
 	uint16_t list[WAVES][64 * ITEMS];
 	uint32_t stage[WAVES][640];
-	cadrMesh meshes[kLdsMeshes];
+	uint32_t meshes[16 * kLdsMeshes];
 	float red[WAVES][6];
 	float red2[WAVES];
 	uint32_t bad[WAVES];

@@ -26,6 +26,19 @@ __device__ __forceinline__ void storeF4(P p, float4 v)
 	p->w = v.w;
 }
 
+// LDS slot of 16-byte unit q (matrix q/4, column q%4) of a wave's staged
+// block.  Columns are rotated within their matrix by ((m >> 1) ^ (m >> 3)) for
+// matrix m, so that both the lane-contiguous accesses (lane l: unit 64k + l)
+// and the per-matrix ones (lane l: units 4l..4l+3, one column at a time)
+// touch distinct 16-byte bank groups in every 8 and every 16 consecutive
+// lanes: no LDS bank conflicts either way (the plain layout puts every
+// second lane of the per-matrix access in the same banks).
+__device__ __forceinline__ uint32_t stageSlot(uint32_t q)
+{
+	const uint32_t m = q >> 2;
+	return (m << 2) | (((q & 3u) + ((m >> 1) ^ (m >> 3))) & 3u);
+}
+
 // A wave's 64 matrices (4 KiB, contiguous in memory) move between HBM and
 // registers through LDS: four 16-byte loads per lane at lane stride 16 B (each
 // instruction one contiguous KiB), then each lane reads its own 64 bytes from
@@ -38,13 +51,13 @@ __device__ __forceinline__ void loadMatrices(Src src, uint64_t first, uint64_t c
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
 		const uint32_t q = uint32_t(k) * 64u + lane;      // float4 index inside the wave's 4 KiB
-		if (first + q / 4u < count) stage[q] = loadF4(src + (4 * first + q));
+		if (first + q / 4u < count) stage[stageSlot(q)] = loadF4(src + (4 * first + q));
 	}
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 	__builtin_amdgcn_wave_barrier();
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-	for (int c = 0; c < 4; c++) M[c] = stage[4 * lane + c];
+	for (int c = 0; c < 4; c++) M[c] = stage[stageSlot(4 * lane + c)];
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 	__builtin_amdgcn_wave_barrier();
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -55,14 +68,14 @@ __device__ __forceinline__ void storeMatrices(Dst dst, uint64_t first, uint64_t 
                                               const float4 M[4])
 {
 #pragma unroll
-	for (int c = 0; c < 4; c++) stage[4 * lane + c] = M[c];
+	for (int c = 0; c < 4; c++) stage[stageSlot(4 * lane + c)] = M[c];
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 	__builtin_amdgcn_wave_barrier();
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
 		const uint32_t q = uint32_t(k) * 64u + lane;
-		if (first + q / 4u < count) storeF4(dst + (4 * first + q), stage[q]);
+		if (first + q / 4u < count) storeF4(dst + (4 * first + q), stage[stageSlot(q)]);
 	}
 }
 
