@@ -407,10 +407,19 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
         log(f"uneven64 autotune: {describe(config)}  {tuned['candidates_ms']}")
     batch = cull.CullBatch(config, entries)
 
+    # one propagation launch per tree depth, over all scenes that have it
+    by_depth = {}
+    for lvs in levels:
+        for d, lv in enumerate(lvs):
+            by_depth.setdefault(d, []).append(lv)
+    depth_batches = []
+    for d in sorted(by_depth):
+        host = np.concatenate(by_depth[d])
+        depth_batches.append((device.DeviceBuffer.from_array(host), host))
+
     def propagate_all():
-        for lvs in levels:
-            for lv in lvs:
-                N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
+        for dev, host in depth_batches:
+            N.check(lib.cadrPropagateTransformsBatch(dev.address, N.ptr(host), host.shape[0], s))
 
     # plain: one single-view launch per (scene, view), same outputs, tiles
     # built for the single-view tile size
@@ -476,8 +485,8 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     cull_gbs = cull_bytes / (cull_ms * 1e-3) / 1e9
     res = {
         "workload": "uneven64: 64 scenes of round(exp(U[ln 1e4, ln 5e6])) drawables, tree depth U{1..4}, LOD count "
-                    "U{1..4}, 64 StateSets, U{1..32} views each; propagation of every scene + all views of all "
-                    "scenes in one persistent cull launch, one HIP graph per frame",
+                    "U{1..4}, 64 StateSets, U{1..32} views each; propagation of every scene (one launch per tree "
+                    "depth) + all views of all scenes in one persistent cull launch, one HIP graph per frame",
         "scenes": len(dss), "drawables": leaves, "nodes": nodes, "views": int(sum(v.shape[0] for v in views_l)),
         "tests_per_frame": tests, "visible": survivors,
         "ms_per_frame": frame_ms, "cull_ms": cull_ms,
@@ -501,6 +510,8 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     batch.close(); ws.close()
     for ds, _, out in entries:
         out.close(); ds.close()
+    for dev, _ in depth_batches:
+        dev.close()
     return res
 
 

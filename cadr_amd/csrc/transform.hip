@@ -15,6 +15,9 @@
 // 64 B), which keeps the world array usable as MatrixList payload; a wave's
 // 64 matrices are staged through LDS so that every global load and store
 // instruction moves one contiguous KiB.
+#include <algorithm>
+#include <string>
+
 #include "internal.h"
 #include "fp32.h"
 #include "matrix_io.h"
@@ -72,6 +75,61 @@ __global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict_
 	}
 }
 
+// Many levels (of different trees) in one launch: the levels' nodes are cut
+// into units of 64 (one wave each), numbered level after level; the grid's
+// waves stride over the units.  Each unit is computed exactly as
+// propagateKernel computes it, with the level's root/sphere choice made per
+// unit (uniform in the wave).  Levels in one launch must not depend on each
+// other; the caller orders dependent levels into successive launches.
+constexpr uint32_t kMaxBatchLevels = 1024;
+
+__global__ __launch_bounds__(256) void propagateBatchKernel(const cadrTransformLevel* __restrict__ levels,
+                                                            uint32_t numLevels)
+{
+	__shared__ float4 stage[4][256];
+	__shared__ uint64_t firstUnit[kMaxBatchLevels + 1];
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	if (threadIdx.x == 0) {
+		uint64_t u = 0;
+		for (uint32_t l = 0; l < numLevels; l++) {
+			firstUnit[l] = u;
+			u += (levels[l].count + 63u) / 64u;
+		}
+		firstUnit[numLevels] = u;
+	}
+	__syncthreads();
+	const uint64_t totalUnits = firstUnit[numLevels];
+	const uint64_t waves = uint64_t(gridDim.x) * 4u;
+	uint32_t l = 0;
+	for (uint64_t unit = uint64_t(blockIdx.x) * 4u + wave; unit < totalUnits; unit += waves) {
+		while (firstUnit[l + 1] <= unit) l++;  // units only grow, so the level index does too
+		const cadrTransformLevel& L = levels[l];
+		const uint64_t count = L.count;
+		const uint64_t first = (unit - firstUnit[l]) * 64u;
+		const uint64_t i = first + lane;
+		const bool live = i < count;
+		const float4* local = reinterpret_cast<const float4*>(L.localMatrices);
+		float4 M[4], W[4];
+		loadMatrices(local, first, count, stage[wave], lane, M);
+		const uint32_t p = (L.parentWorld && live) ? reinterpret_cast<const uint32_t*>(L.parentIndex)[i] : ~0u;
+		if (p < L.parentCount) {
+			const float4* pw = reinterpret_cast<const float4*>(L.parentWorld);
+			float4 P[4];
+#pragma unroll
+			for (int c = 0; c < 4; c++) P[c] = pw[4 * uint64_t(p) + c];
+			mat4Mul(P, M, W);
+		} else {
+#pragma unroll
+			for (int c = 0; c < 4; c++) W[c] = M[c];
+		}
+		storeMatrices(reinterpret_cast<float4*>(L.worldMatrices), first, count, stage[wave], lane, W);
+		if (L.spheres && live)
+			reinterpret_cast<float4*>(L.spheres)[i] =
+			    nodeSphere(W, reinterpret_cast<const cadrMesh*>(L.meshes), L.numMeshes,
+			               reinterpret_cast<const uint32_t*>(L.meshIndex)[i]);
+	}
+}
+
 __global__ __launch_bounds__(256) void worldSpheresKernel(const float4* __restrict__ world,
                                                           const uint32_t* __restrict__ meshIndex,
                                                           const cadrMesh* __restrict__ meshes, uint32_t numMeshes,
@@ -117,6 +175,41 @@ CADR_API int cadrPropagateTransforms(const cadrTransformLevel* lv, cadrStream st
 	else if (spheres) hipLaunchKernelGGL((propagateKernel<false, true>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
 	else hipLaunchKernelGGL((propagateKernel<false, false>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
 	CADR_CHECK_HIP(hipGetLastError(), "propagateKernel launch");
+	return ok();
+}
+
+namespace {
+int checkLevel(const cadrTransformLevel* lv, const char* who)
+{
+	std::string w(who);
+	CADR_REQUIRE(lv->localMatrices && lv->worldMatrices, (w + ": null matrix array").c_str());
+	CADR_REQUIRE((lv->localMatrices % 16) == 0 && (lv->worldMatrices % 16) == 0 && (lv->parentWorld % 16) == 0,
+	             (w + ": matrix arrays must be 16-byte aligned").c_str());
+	CADR_REQUIRE(lv->parentWorld == 0 || lv->parentIndex, (w + ": parentIndex is NULL for a non-root level").c_str());
+	CADR_REQUIRE(lv->spheres == 0 || (lv->meshIndex && lv->meshes && (lv->spheres % 16) == 0 && (lv->meshes % 16) == 0),
+	             (w + ": spheres need meshIndex and 16-byte aligned meshes/spheres").c_str());
+	return CADR_SUCCESS;
+}
+}  // namespace
+
+CADR_API int cadrPropagateTransformsBatch(uint64_t deviceLevels, const cadrTransformLevel* hostLevels,
+                                          uint32_t numLevels, cadrStream stream)
+{
+	CADR_REQUIRE(numLevels == 0 || (hostLevels && deviceLevels), "cadrPropagateTransformsBatch: null argument");
+	CADR_REQUIRE(numLevels <= kMaxBatchLevels, "cadrPropagateTransformsBatch: more than 1024 levels");
+	CADR_REQUIRE(deviceLevels % 8 == 0, "cadrPropagateTransformsBatch: misaligned level array");
+	uint64_t units = 0;
+	for (uint32_t l = 0; l < numLevels; l++) {
+		if (hostLevels[l].count == 0) continue;
+		const int rc = checkLevel(&hostLevels[l], "cadrPropagateTransformsBatch");
+		if (rc != CADR_SUCCESS) return rc;
+		units += (hostLevels[l].count + 63u) / 64u;
+	}
+	if (units == 0) return ok();
+	const uint64_t blocks = std::min<uint64_t>((units + 3) / 4, uint64_t(computeUnitCount()) * 16u);
+	hipLaunchKernelGGL(propagateBatchKernel, dim3(uint32_t(blocks)), dim3(256), 0, hipStream_t(stream),
+	                   reinterpret_cast<const cadrTransformLevel*>(deviceLevels), numLevels);
+	CADR_CHECK_HIP(hipGetLastError(), "propagateBatchKernel launch");
 	return ok();
 }
 

@@ -221,6 +221,15 @@ typedef struct cadrTransformLevel {
 
 int cadrPropagateTransforms(const cadrTransformLevel* level, cadrStream stream);
 
+/* Many independent levels (for example depth d of many trees) in one launch:
+ * each computed exactly as cadrPropagateTransforms computes it.  deviceLevels
+ * holds cadrTransformLevel[numLevels] in device memory, hostLevels the same
+ * descriptors on the host (for checks).  No level may read another level of
+ * the same launch; dependent levels go to successive launches on a stream.
+ * numLevels <= 1024.  Capturable in a HIP graph. */
+int cadrPropagateTransformsBatch(uint64_t deviceLevels, const cadrTransformLevel* hostLevels, uint32_t numLevels,
+                                 cadrStream stream);
+
 /* World spheres of drawables whose world matrices are already known (flat
  * scenes): spheres[i] = worldMatrices[i] * meshes[meshIndex[i]].sphere. */
 int cadrComputeWorldSpheres(uint64_t worldMatrices, uint64_t meshIndex, uint64_t meshes,

@@ -299,3 +299,33 @@ def test_fused_rejects_bad_arguments(gpu):
     with pytest.raises(N.CadrError):
         b.run_fused(bad)
     b.close(); out.close(); ds.close()
+
+
+def test_batched_propagation_matches_per_level(gpu):
+    """cadrPropagateTransformsBatch (one launch per tree depth over several
+    trees) against the oracle: world matrices and spheres bit for bit."""
+    from cadr_amd.device import DeviceBuffer, synchronize
+    specs = [(3000, 1, 1), (50_000, 2, 4), (120_000, 3, 2), (1, 1, 1), (70_000, 4, 3), (6_500, 3, 3)]
+    sc_list = [scenes.uneven_scene(40 + i, n, depth, lods, 1) for i, (n, depth, lods) in enumerate(specs)]
+    dss = [cull.DeviceScene(sc) for sc in sc_list]
+    by_depth = {}
+    for ds in dss:
+        for d, lv in enumerate(ds.level_descs(spheres=True)):
+            by_depth.setdefault(d, []).append(lv)
+    bufs = []
+    for d in sorted(by_depth):
+        host = np.concatenate(by_depth[d])
+        dev = DeviceBuffer.from_array(host)
+        bufs.append(dev)
+        N.check(N.load().cadrPropagateTransformsBatch(dev.address, N.ptr(host), host.shape[0], None))
+    synchronize()
+    for sc, ds in zip(sc_list, dss):
+        worlds, sph = oracle.propagate_scene(sc)
+        for d in range(len(worlds)):
+            assert ds.world_matrices(d).tobytes() == worlds[d].tobytes(), f"{sc.name} level {d}"
+        assert ds.download_spheres().tobytes() == sph.tobytes(), f"{sc.name} spheres"
+        ds.close()
+    for b in bufs:
+        b.close()
+    with pytest.raises(N.CadrError):
+        N.check(N.load().cadrPropagateTransformsBatch(0, None, 1, None))
