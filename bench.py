@@ -357,6 +357,7 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
         "culled_drawables_per_s": tests / (batch_ms * 1e-3),
         "roofline": {"bound": "hbm", "kernel": f"cullKernel<{config.blockSize},{config.itemsPerThread}>",
                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"cullKernel<{config.blockSize},{config.itemsPerThread}>"),
                      "algorithmic_bytes": alg_bytes},
         "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags,
                    "autotune": tuned},
@@ -639,7 +640,8 @@ def bench_gather(args, stream, lib, N, scenes, device, n=10_000_000):
                        "MatrixList per drawable, 4 PrimitiveSets per geometry)",
            "ms": ms, "drawables_per_s": n / (ms * 1e-3),
            "roofline": {"bound": "hbm", "kernel": f"processDrawablesKernel<{level}>", "achieved": gbs,
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": alg},
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": alg,
+                        "traffic": measured_traffic(f"processDrawablesKernel<{level}>")},
            "outputs_match_closed_form": same,
            "cpu_oracle": {"value": cpu_rate, "unit": "drawables/s", "cores": 1,
                           "sample": f"{m} drawables, handle level 3, oracle/cadr_oracle.c, best of 3"}}
@@ -701,7 +703,7 @@ class Graph:
         self.exec = self.graph = None
 
 
-def cpu_baseline(scale: float = 0.25, budget_s: float = 10.0):
+def cpu_baseline(scale: float = 0.25, budget_s: float = 12.0):
     """The oracle (C restatement, gcc -O2, one thread) on a bounded sample of
     the workload: the full10m structure with a quarter of the roots (2.5M
     drawables), whole frames (propagation + cull of one view) repeated until
@@ -713,7 +715,7 @@ def cpu_baseline(scale: float = 0.25, budget_s: float = 10.0):
     views = cull.make_views([scenes.DEFAULT_CAMERA])
     times = []
     start = time.perf_counter()
-    while len(times) < 3 or (time.perf_counter() - start < budget_s and len(times) < 50):
+    while len(times) < 3 or (time.perf_counter() - start < budget_s and len(times) < 500):
         t0 = time.perf_counter()
         _, spheres = oracle.propagate_scene(scene)
         oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views, cmd_dtype=N.CMD_DTYPE,

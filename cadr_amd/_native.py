@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcadr_hip.so")
+# CADR_HIP_LIBRARY selects another build of the library (same C ABI)
+LIB_PATH = os.environ.get("CADR_HIP_LIBRARY") or os.path.join(_HERE, "libcadr_hip.so")
 
 c_u64 = ctypes.c_uint64
 c_u32 = ctypes.c_uint32

@@ -89,15 +89,26 @@ __global__ __launch_bounds__(256) void propagateBatchKernel(const cadrTransformL
 	__shared__ float4 stage[4][256];
 	__shared__ uint64_t firstUnit[kMaxBatchLevels + 1];
 	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-	if (threadIdx.x == 0) {
-		uint64_t u = 0;
-		for (uint32_t l = 0; l < numLevels; l++) {
-			firstUnit[l] = u;
-			u += (levels[l].count + 63u) / 64u;
-		}
-		firstUnit[numLevels] = u;
-	}
+	// firstUnit[l] = units of levels 0..l-1: loaded in parallel, then an
+	// in-place Hillis-Steele scan over the (at most 1024) levels
+	for (uint32_t i = threadIdx.x; i < numLevels; i += 256u) firstUnit[i + 1] = (levels[i].count + 63u) / 64u;
+	if (threadIdx.x == 0) firstUnit[0] = 0;
 	__syncthreads();
+	for (uint32_t off = 1; off < numLevels; off <<= 1) {
+		uint64_t add[kMaxBatchLevels / 256];
+#pragma unroll
+		for (uint32_t k = 0; k < kMaxBatchLevels / 256; k++) {
+			const uint32_t i = threadIdx.x + 256u * k;
+			add[k] = (i < numLevels && i >= off) ? firstUnit[i + 1 - off] : 0u;
+		}
+		__syncthreads();
+#pragma unroll
+		for (uint32_t k = 0; k < kMaxBatchLevels / 256; k++) {
+			const uint32_t i = threadIdx.x + 256u * k;
+			if (i < numLevels) firstUnit[i + 1] += add[k];
+		}
+		__syncthreads();
+	}
 	const uint64_t totalUnits = firstUnit[numLevels];
 	const uint64_t waves = uint64_t(gridDim.x) * 4u;
 	uint32_t l = 0;
