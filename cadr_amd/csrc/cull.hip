@@ -19,19 +19,21 @@
 // over StateSets, so the walks stay short.  Status words are tagged with a
 // launch epoch and never cleared.
 //
-// Two kernels:
+// Three kernels:
 //   cullViewKernel - one view: one workgroup per tile, visibility kept in
-//     registers, a 64-lane parallel look-back in wave 0, direct 20-byte
-//     writes.  It resets its ticket and advances the epoch itself (the last
-//     workgroup to finish), so a view costs one launch.
+//     registers, a 64-lane parallel look-back in wave 0, commands staged in
+//     LDS and stored as whole runs.  It resets its ticket and advances the
+//     epoch itself (the last workgroup to finish), so a view costs one launch.
+//   cullFusedKernel - the same for one view, with the tile's spheres computed
+//     from the last transform level instead of read (cadrPropagateCullView).
 //   cullKernel - many views and many scenes (jobs) in one persistent launch:
-//     a tile is read from HBM once, staged in LDS and tested against all views
-//     of its job, 64 views at a time; lane v of wave 0 walks the chain of view
-//     v; writes are compacted per wave through an LDS index list, so lane l
-//     stores the l-th visible drawable.  A conservative bounding sphere of the
-//     tile, classified per view with lanes as views, lets whole (tile, view)
-//     pairs skip the per-drawable test when the tile is wholly outside or
-//     wholly inside the frustum.
+//     a tile is read from HBM once into registers and tested against all
+//     views of its job, 64 views at a time; lane v of wave 0 walks the chain
+//     of view v; each wave compacts its visible drawables through a staging
+//     ring in LDS.  A conservative bounding sphere of the tile, classified
+//     per view with lanes as views, lets whole (tile, view) pairs skip the
+//     per-drawable test when the tile is wholly outside or inside the
+//     frustum.
 #include "internal.h"
 #include "fp32.h"
 #include "matrix_io.h"
@@ -697,12 +699,11 @@ template <int BLOCK, int ITEMS>
 struct CullShared {
 	static constexpr int WAVES = BLOCK / 64;
 	static constexpr int TILE = BLOCK * ITEMS;
-	float4 sph[TILE];
-	uint32_t mesh[TILE];
+
 	uint64_t mask[kViewGroup][WAVES][ITEMS];
 	uint32_t cnt[kViewGroup][WAVES];
 	uint32_t excl[kViewGroup];
-	uint16_t list[WAVES][64 * ITEMS];
+
 	uint32_t stage[WAVES][640];
 	uint32_t meshes[16 * kLdsMeshes];
 	float red[WAVES][6];
@@ -769,8 +770,7 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			sp[j] = s;
 			sp[j] = s;
 			mi[j] = m;
-			sh.sph[e] = s;
-			sh.mesh[e] = m;
+
 			// a sphere other than a finite one with r >= 0 or the empty sphere
 			// (r = -inf, never visible) disables the tile sphere for this tile
 			if (e < T.count && s.w != -__builtin_inff() &&
@@ -896,7 +896,10 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			}
 			__syncthreads();
 
-			// 3. compacted writes: lane r of the wave stores its r-th visible drawable
+			// 3. compacted writes: each item's visible drawables go straight from
+			// registers into the wave's 128-command staging ring (at the rank of
+			// the lane among the visible ones), and the ring is stored 64
+			// commands at a time as five 256-byte runs
 			auto writeViews = [&](auto lds) {
 				for (uint64_t am = active; am != 0ull; am &= am - 1ull) {
 					const uint32_t v = uint32_t(__builtin_ctzll(am));
@@ -904,79 +907,48 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 					if (run == 0) continue;
 					uint32_t wavePos = sh.excl[v];
 					for (uint32_t q = 0; q < wave; q++) wavePos += sh.cnt[v][q];
-					wavePos = uniform(wavePos);
 					const ViewC V = loadView(views, g0 + v);
 					const gmut<uint32_t> out = globalMut<uint32_t>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
-					if (4u * run >= uint32_t(WAVE_ITEMS)) {
-						// dense (a quarter or more of the wave visible): each item's
-						// visible drawables go straight from registers into a
-						// 128-command staging ring, stored 64 at a time
-						uint32_t* stage = sh.stage[wave];
-						uint32_t pos = T.segBase + wavePos;  // output slot of stage[0]
-						uint32_t pend = 0;                   // staged, not yet stored
-#pragma unroll
-						for (int j = 0; j < ITEMS; j++) {
-							const uint64_t m = sh.mask[v][wave][j];
-							const uint32_t n = uint32_t(__builtin_popcountll(m));
-							if (n == 0) continue;
-							if ((m >> lane) & 1ull) {
-								uint32_t count, first, vofs;
-								meshCommand<decltype(lds)::value>(meshes, mi[j], sp[j].w, viewCoord(V.r2, sp[j]), V,
-								                                  count, first, vofs);
-								stageCommand(stage, pend + maskRank(m), count, first, vofs,
-								             T.start + wave * WAVE_ITEMS + j * 64 + lane);
-							}
-							pend += n;
-							if (pend >= 64u) {
-								waveSync();
-								flushCommands(out, stage, pos, 64u, J.commandStride, lane);
-								pos += 64u;
-								pend -= 64u;
-								// the overflow (fewer than 64 commands) moves to the front
-								uint32_t carry[5];
-#pragma unroll
-								for (uint32_t k = 0; k < 5; k++) {
-									const uint32_t d = k * 64u + lane;
-									carry[k] = d < 5u * pend ? stage[320u + d] : 0u;
-								}
-								waveSync();
-#pragma unroll
-								for (uint32_t k = 0; k < 5; k++) {
-									const uint32_t d = k * 64u + lane;
-									if (d < 5u * pend) stage[d] = carry[k];
-								}
-								waveSync();
-							}
-						}
-						if (pend > 0u) {
-							waveSync();
-							flushCommands(out, stage, pos, pend, J.commandStride, lane);
-							waveSync();
-						}
-						continue;
-					}
-					// otherwise gather the visible drawables of all items into a
-					// list, then write them 64 at a time
-					uint32_t r0 = 0;
+					uint32_t* stage = sh.stage[wave];
+					uint32_t pos = uniform(T.segBase + wavePos);  // output slot of stage[0]
+					uint32_t pend = 0;                            // staged, not yet stored
 #pragma unroll
 					for (int j = 0; j < ITEMS; j++) {
 						const uint64_t m = sh.mask[v][wave][j];
-						if ((m >> lane) & 1ull) sh.list[wave][r0 + maskRank(m)] = uint16_t(j * 64 + lane);
-						r0 += uint32_t(__builtin_popcountll(m));
-					}
-					waveSync();
-					for (uint32_t r0 = 0; r0 < run; r0 += 64) {
-						const uint32_t n = min(64u, run - r0);
-						if (lane < n) {
-							const uint32_t e = wave * WAVE_ITEMS + sh.list[wave][r0 + lane];
-							const float4 s = sh.sph[e];
+						const uint32_t n = uint32_t(__builtin_popcountll(m));
+						if (n == 0) continue;
+						if ((m >> lane) & 1ull) {
 							uint32_t count, first, vofs;
-							meshCommand<decltype(lds)::value>(meshes, sh.mesh[e], s.w, viewCoord(V.r2, s), V, count, first,
-							                                  vofs);
-							stageCommand(sh.stage[wave], lane, count, first, vofs, T.start + e);
+							meshCommand<decltype(lds)::value>(meshes, mi[j], sp[j].w, viewCoord(V.r2, sp[j]), V, count,
+							                                  first, vofs);
+							stageCommand(stage, pend + maskRank(m), count, first, vofs,
+							             T.start + wave * WAVE_ITEMS + j * 64 + lane);
 						}
+						pend += n;
+						if (pend >= 64u) {
+							waveSync();
+							flushCommands(out, stage, pos, 64u, J.commandStride, lane);
+							pos += 64u;
+							pend -= 64u;
+							// the overflow (fewer than 64 commands) moves to the front
+							uint32_t carry[5];
+#pragma unroll
+							for (uint32_t k = 0; k < 5; k++) {
+								const uint32_t d = k * 64u + lane;
+								carry[k] = d < 5u * pend ? stage[320u + d] : 0u;
+							}
+							waveSync();
+#pragma unroll
+							for (uint32_t k = 0; k < 5; k++) {
+								const uint32_t d = k * 64u + lane;
+								if (d < 5u * pend) stage[d] = carry[k];
+							}
+							waveSync();
+						}
+					}
+					if (pend > 0u) {
 						waveSync();
-						flushCommands(out, sh.stage[wave], T.segBase + wavePos + r0, n, J.commandStride, lane);
+						flushCommands(out, stage, pos, pend, J.commandStride, lane);
 						waveSync();
 					}
 				}
