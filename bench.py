@@ -158,8 +158,8 @@ def main():
     if not args.no_tune:
         propagate_inner()
         fused_table = {}
-        for items in (2, 4, 8, 16):
-            cfg = cull.make_config(256, items)
+        for block, items in ((256, 2), (256, 4), (256, 8), (256, 16), (512, 2), (512, 4), (1024, 1), (1024, 2)):
+            cfg = cull.make_config(block, items)
             b = cull.CullBatch(cfg, [(ds, views1, out1)])
             b.run_fused(leaf_fused, s)
             e0, e1 = Event(), Event()
@@ -168,10 +168,10 @@ def main():
                 b.run_fused(leaf_fused, s)
             e1.record(s)
             stream.synchronize()
-            fused_table[f"256x{items}"] = e0.elapsed_ms(e1) / 5
+            fused_table[f"{block}x{items}"] = e0.elapsed_ms(e1) / 5
             b.close()
         best = min(fused_table, key=fused_table.get)
-        fused_config = cull.make_config(256, int(best.split("x")[1]))
+        fused_config = cull.make_config(*(int(x) for x in best.split("x")))
         log(f"fused autotune: {best}  {fused_table}")
     batch_fused = cull.CullBatch(fused_config, [(ds, views1, out1)])
 
@@ -249,10 +249,10 @@ def main():
                    "frame": "2 propagation launches (root, inner level) + 1 fused leaf propagation + cull launch",
                    "block": fused_config.blockSize, "items_per_thread": fused_config.itemsPerThread,
                    "autotune": {"fused_ms": fused_table, "single_view": tuned1}},
-        "roofline": {"bound": "hbm", "kernel": f"cullFusedKernel<{fused_config.itemsPerThread}> "
+        "roofline": {"bound": "hbm", "kernel": f"cullFusedKernel<{fused_config.blockSize},{fused_config.itemsPerThread}> "
                                                "(leaf propagation + cull + LOD + compaction)",
                      "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fused_gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"cullFusedKernel<{fused_config.itemsPerThread}>"),
+                     "traffic": measured_traffic(f"cullFusedKernel<{fused_config.blockSize},{fused_config.itemsPerThread}>"),
                      "algorithmic_bytes": fused_bytes, "avg_ms": fused_ms},
         "frame": {"algorithmic_bytes": frame_bytes, "hbm_frac": frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                   "visible": survivors1, "counts_identical_to_unfused": same},

@@ -562,13 +562,13 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 
 // ---------------------------------------------------------------- one view
 
-template <int ITEMS, bool LDS, bool FUSED>
+template <int BLOCK, int ITEMS, bool LDS, bool FUSED>
 __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCullJob& J, uint32_t t, uint32_t epoch,
                                              const MeshTable& meshes, uint32_t (*sCnt)[ITEMS], uint32_t* sExclP,
                                              uint32_t* sStage, float4* sMat, uint32_t tid, uint32_t lane,
                                              uint32_t wave)
 {
-	constexpr int BLOCK = 256, WAVES = 4, WAVE_ITEMS = 64 * ITEMS;
+	constexpr int WAVES = BLOCK / 64, WAVE_ITEMS = 64 * ITEMS;
 	const TileDesc T = loadTile(J, t, BLOCK * ITEMS);
 	const ViewC V = loadView(J.views, 0);
 	const gptr<float4> spheres = globalPtr<float4>(J.spheres);
@@ -638,10 +638,10 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 }
 
 
-template <int ITEMS, bool FUSED>
+template <int BLOCK, int ITEMS, bool FUSED>
 __device__ __forceinline__ void cullViewBody(const CullParams& P)
 {
-	constexpr int BLOCK = 256, WAVES = 4;
+	constexpr int WAVES = BLOCK / 64;
 	__shared__ float4 sMat[FUSED ? WAVES : 1][FUSED ? 256 : 1];
 	__shared__ uint32_t sTicket;
 	__shared__ uint32_t sCnt[WAVES][ITEMS];
@@ -660,9 +660,10 @@ __device__ __forceinline__ void cullViewBody(const CullParams& P)
 	if (t < P.totalWork) {
 		float4* mat = &sMat[FUSED ? wave : 0][0];
 		if (ldsMeshes)
-			cullViewTile<ITEMS, true, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, mat, tid, lane, wave);
+			cullViewTile<BLOCK, ITEMS, true, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, mat, tid, lane, wave);
 		else
-			cullViewTile<ITEMS, false, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, mat, tid, lane, wave);
+			cullViewTile<BLOCK, ITEMS, false, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, mat, tid, lane,
+			                                          wave);
 	}
 
 	// the last workgroup to finish resets the ticket and starts a new epoch
@@ -681,16 +682,17 @@ __device__ __forceinline__ void cullViewBody(const CullParams& P)
 template <int ITEMS>
 __global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
 {
-	cullViewBody<ITEMS, false>(P);
+	cullViewBody<256, ITEMS, false>(P);
 }
 
 // The fused kernel is latency-bound (a chain of dependent loads per item), so
 // occupancy pays: WPE waves per SIMD is requested from the register allocator
-// (4 for the small item counts, where it costs no spills).
-template <int ITEMS, int WPE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void cullFusedKernel(CullParams P)
+// (4 for the small item counts, where it costs no spills).  Larger workgroups
+// give each wave fewer items, so the chain per tile is shorter.
+template <int BLOCK, int ITEMS, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void cullFusedKernel(CullParams P)
 {
-	cullViewBody<ITEMS, true>(P);
+	cullViewBody<BLOCK, ITEMS, true>(P);
 }
 
 // ------------------------------------------------------- many views, jobs
@@ -1048,12 +1050,14 @@ struct KernelEntry {
 };
 
 const KernelEntry kKernels[] = {
-    {256, 2, cullKernel<256, 2>, cullViewKernel<2>, cullFusedKernel<2, 4>},
-    {256, 4, cullKernel<256, 4>, cullViewKernel<4>, cullFusedKernel<4, 4>},
-    {256, 8, cullKernel<256, 8>, cullViewKernel<8>, cullFusedKernel<8, 1>},
-    {256, 16, cullKernel<256, 16>, cullViewKernel<16>, cullFusedKernel<16, 1>},
-    {512, 2, cullKernel<512, 2>, nullptr, nullptr},
-    {512, 4, cullKernel<512, 4>, nullptr, nullptr},
+    {256, 2, cullKernel<256, 2>, cullViewKernel<2>, cullFusedKernel<256, 2, 4>},
+    {256, 4, cullKernel<256, 4>, cullViewKernel<4>, cullFusedKernel<256, 4, 4>},
+    {256, 8, cullKernel<256, 8>, cullViewKernel<8>, cullFusedKernel<256, 8, 1>},
+    {256, 16, cullKernel<256, 16>, cullViewKernel<16>, cullFusedKernel<256, 16, 1>},
+    {512, 2, cullKernel<512, 2>, nullptr, cullFusedKernel<512, 2, 1>},
+    {512, 4, cullKernel<512, 4>, nullptr, cullFusedKernel<512, 4, 1>},
+    {1024, 1, nullptr, nullptr, cullFusedKernel<1024, 1, 1>},
+    {1024, 2, nullptr, nullptr, cullFusedKernel<1024, 2, 1>},
     {128, 8, cullKernel<128, 8>, nullptr, nullptr},
     {128, 16, cullKernel<128, 16>, nullptr, nullptr},
 };
@@ -1173,7 +1177,7 @@ CADR_API int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, co
                            uint32_t numJobs, cadrCullWorkspace* ws, cadrStream stream)
 {
 	const KernelEntry* ke = findKernel(config);
-	CADR_REQUIRE(ke, "cadrCullBatch: unsupported blockSize/itemsPerThread");
+	CADR_REQUIRE(ke && ke->batch, "cadrCullBatch: unsupported blockSize/itemsPerThread");
 	CADR_REQUIRE(ws, "cadrCullBatch: workspace is NULL");
 	if (numJobs == 0) return ok();
 	CADR_REQUIRE(deviceJobs && hostJobs, "cadrCullBatch: null job array");
@@ -1252,7 +1256,8 @@ CADR_API int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCul
 	CADR_REQUIRE(leaf && hostJob, "cadrPropagateCullView: null argument");
 	CADR_REQUIRE(hostJob->numViews == 1, "cadrPropagateCullView: the job must hold exactly one view");
 	const KernelEntry* ke = findKernel(config);
-	CADR_REQUIRE(ke && ke->fused, "cadrPropagateCullView: the config needs blockSize 256 and itemsPerThread 2/4/8/16");
+	CADR_REQUIRE(ke && ke->fused,
+	             "cadrPropagateCullView: unsupported blockSize/itemsPerThread (256 x 2/4/8/16, 512 x 2/4, 1024 x 1/2)");
 	CADR_REQUIRE(ws, "cadrPropagateCullView: workspace is NULL");
 	CADR_REQUIRE(deviceJob && deviceJob % 16 == 0, "cadrPropagateCullView: deviceJob must be a 16-byte aligned address");
 	CADR_REQUIRE(leaf->count == hostJob->numDrawables, "cadrPropagateCullView: the leaf level must hold the job's drawables");
@@ -1278,7 +1283,7 @@ CADR_API int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCul
 	p.flags = config->flags;
 	p.leaf = LeafParams{leaf->localMatrices, leaf->parentIndex, leaf->parentWorld, leaf->worldMatrices, leaf->spheres,
 	                    leaf->parentWorld ? leaf->parentCount : 0u};
-	hipLaunchKernelGGL(ke->fused, dim3(hostJob->numTiles), dim3(256), 0, hipStream_t(stream), p);
+	hipLaunchKernelGGL(ke->fused, dim3(hostJob->numTiles), dim3(ke->block), 0, hipStream_t(stream), p);
 	CADR_CHECK_HIP(hipGetLastError(), "fused cullViewKernel launch");
 	return ok();
 }

@@ -111,10 +111,14 @@ __global__ __launch_bounds__(256) void propagateBatchKernel(const cadrTransformL
 	}
 	const uint64_t totalUnits = firstUnit[numLevels];
 	const uint64_t waves = uint64_t(gridDim.x) * 4u;
-	uint32_t l = 0;
+	uint32_t l = 0, cached = ~0u;
+	cadrTransformLevel L{};
 	for (uint64_t unit = uint64_t(blockIdx.x) * 4u + wave; unit < totalUnits; unit += waves) {
 		while (firstUnit[l + 1] <= unit) l++;  // units only grow, so the level index does too
-		const cadrTransformLevel& L = levels[l];
+		if (l != cached) {  // the descriptor stays in scalar registers while the level lasts
+			L = levels[l];
+			cached = l;
+		}
 		const uint64_t count = L.count;
 		const uint64_t first = (unit - firstUnit[l]) * 64u;
 		const uint64_t i = first + lane;

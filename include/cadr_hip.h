@@ -361,7 +361,8 @@ int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCul
  * drawable i of the job (leaf->count == numDrawables); the spheres are made
  * from the job's meshIndex and meshes (the leaf's own meshIndex/meshes are
  * not read).  World matrices, spheres, commands and counts are bit-identical
- * to the two separate calls.  config: blockSize 256, itemsPerThread 2/4/8/16. */
+ * to the two separate calls.  config (blockSize x itemsPerThread): 256 x 2/4/8/16,
+ * 512 x 2/4 or 1024 x 1/2 (the last two shapes exist for this entry point only). */
 int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCullConfig* config, uint64_t deviceJob,
                           const cadrCullJob* hostJob, cadrCullWorkspace* workspace, cadrStream stream);
 

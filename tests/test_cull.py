@@ -231,8 +231,9 @@ def test_cull_view_after_batch_shares_workspace(plain):
     ws.close()
 
 
-@pytest.mark.parametrize("items", [2, 4, 8, 16])
-def test_fused_propagation_and_cull(gpu, items):
+@pytest.mark.parametrize("block,items", [(256, 2), (256, 4), (256, 8), (256, 16), (512, 2), (512, 4), (1024, 1),
+                                         (1024, 2)])
+def test_fused_propagation_and_cull(gpu, block, items):
     """cadrPropagateCullView against cadrPropagateTransforms + cadrCullView and the
     oracle: world matrices, spheres, commands and counts bit for bit; tree and
     flat (root) leaf levels, a mesh table too large for LDS, spheres on and off."""
@@ -246,9 +247,9 @@ def test_fused_propagation_and_cull(gpu, items):
     ]
     cam = scenes.Camera(eye=(10.0, -20.0, 150.0), dir=(-0.1, 0.1, -1.0), up=(0.0, 1.0, 0.0), fovy=1.1)
     views = cull.make_views([cam])
-    config = cull.make_config(256, items)
+    config = cull.make_config(block, items)
     for name, sc in cases:
-        print(f"fused {name} items={items}", flush=True)
+        print(f"fused {name} {block}x{items}", flush=True)
         worlds, sph = oracle.propagate_scene(sc)
         exp_cmds, exp_counts = oracle.cull(sph, sc.mesh_index, sc.meshes, sc.stateset_counts, views,
                                            cmd_dtype=N.CMD_DTYPE)
@@ -285,7 +286,7 @@ def test_fused_rejects_bad_arguments(gpu):
     views = cull.make_views([scenes.DEFAULT_CAMERA])
     out = cull.CullOutputs(1, ds.n, ds.K)
     leaf = ds.level_descs()[-1].copy()
-    for cfg in (cull.make_config(512, 4), cull.make_config(256, 3)):
+    for cfg in (cull.make_config(128, 8), cull.make_config(256, 3)):
         b = cull.CullBatch(cull.make_config(256, 4), [(ds, views, out)])
         b.config = cfg
         with pytest.raises(N.CadrError):
