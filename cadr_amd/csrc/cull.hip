@@ -29,8 +29,8 @@
 //   cullKernel - many views and many scenes (jobs) in one persistent launch:
 //     a tile is read from HBM once into registers and tested against all
 //     views of its job, 64 views at a time; lane v of wave 0 walks the chain
-//     of view v; each wave compacts its visible drawables through a staging
-//     ring in LDS.  A conservative bounding sphere of the tile, classified
+//     of view v; each visible drawable's lane stores its command directly at
+//     its compacted slot.  A conservative bounding sphere of the tile, classified
 //     per view with lanes as views, lets whole (tile, view) pairs skip the
 //     per-drawable test when the tile is wholly outside or inside the
 //     frustum.
@@ -218,11 +218,16 @@ __device__ __forceinline__ bool sphereVisible(float4 s, const ViewC& v)
 	return nearOk & farOk & xOk & yOk;
 }
 
-__device__ __forceinline__ uint32_t lodFromDepth(float r, float vz, const ViewC& v, uint32_t lodCount)
+// Highest LOD index of a mesh with lodCount LODs (lodCount clamped to 1..4).
+__device__ __forceinline__ uint32_t maxLodOf(uint32_t lodCount)
+{
+	return (lodCount >= 1u ? (lodCount <= 4u ? lodCount : 4u) : 1u) - 1u;
+}
+
+__device__ __forceinline__ uint32_t lodAtDepth(float r, float vz, const ViewC& v, uint32_t maxLod)
 {
 	const float p = r * v.lodScale;
 	const uint32_t lod = uint32_t(p < v.th0 * vz) + uint32_t(p < v.th1 * vz) + uint32_t(p < v.th2 * vz);
-	const uint32_t maxLod = (lodCount >= 1u ? (lodCount <= 4u ? lodCount : 4u) : 1u) - 1u;
 	return lod < maxLod ? lod : maxLod;
 }
 
@@ -280,6 +285,15 @@ __device__ __forceinline__ uint32_t maskRank(uint64_t m)
 }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// A 64-bit value that every lane holds alike (an LDS word all lanes read),
+// moved to scalar registers so that tests on it are scalar branches.
+__device__ __forceinline__ uint64_t uniform64(uint64_t x)
+{
+	const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+	const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+	return (uint64_t(hi) << 32) | uint64_t(lo);
+}
 
 __device__ __forceinline__ void waveSync()
 {
@@ -364,7 +378,8 @@ __device__ uint32_t lookBackWave(const uint64_t* chain, uint32_t c, uint32_t epo
 // 64-byte records every mesh's copy of a field would sit in the same two LDS
 // banks, and 64 lanes reading random meshes would serialise.
 // cadrMesh dwords: 0-2 centre, 3 radius, 4 lodCount, 5 vertexOffset,
-// 8-11 indexCount[4], 12-15 firstIndex[4].
+// 8-11 indexCount[4], 12-15 firstIndex[4]; the LDS copy holds the highest
+// LOD index (maxLodOf(lodCount)) in place of dword 4.
 struct MeshTable {
 	const uint32_t* lds;
 	gptr<cadrMesh> mem;
@@ -382,13 +397,13 @@ __device__ __forceinline__ void meshCommand(const MeshTable& M, uint32_t m, floa
                                             uint32_t& count, uint32_t& first, uint32_t& vofs)
 {
 	if constexpr (LDS) {
-		const uint32_t lod = lodFromDepth(r, vz, V, M.field(4, m));
+		const uint32_t lod = lodAtDepth(r, vz, V, M.field(4, m));
 		count = M.field(8 + lod, m);
 		first = M.field(12 + lod, m);
 		vofs = M.field(5, m);
 	} else {
 		const gptr<cadrMesh> mesh = M.mem + m;
-		const uint32_t lod = lodFromDepth(r, vz, V, mesh->lodCount);
+		const uint32_t lod = lodAtDepth(r, vz, V, maxLodOf(mesh->lodCount));
 		count = mesh->indexCount[lod];
 		first = mesh->firstIndex[lod];
 		vofs = uint32_t(mesh->vertexOffset);
@@ -448,7 +463,10 @@ __device__ __forceinline__ bool stageMeshes(uint32_t* sMeshes, const cadrCullJob
 {
 	if (J.numMeshes > kLdsMeshes) return false;
 	const gptr<uint32_t> src = globalPtr<uint32_t>(J.meshes);
-	for (uint32_t i = tid; i < J.numMeshes * 16u; i += BLOCK) sMeshes[(i % 16u) * kLdsMeshes + i / 16u] = src[i];
+	for (uint32_t i = tid; i < J.numMeshes * 16u; i += BLOCK) {
+		const uint32_t x = src[i];
+		sMeshes[(i % 16u) * kLdsMeshes + i / 16u] = i % 16u == 4u ? maxLodOf(x) : x;
+	}
 	return true;
 }
 
@@ -704,9 +722,8 @@ struct CullShared {
 
 	uint64_t mask[kViewGroup][WAVES][ITEMS];
 	uint32_t cnt[kViewGroup][WAVES];
-	uint32_t excl[kViewGroup];
+	uint32_t base[kViewGroup][WAVES];  // a wave's first output slot in its view's StateSet segment
 
-	uint32_t stage[WAVES][640];
 	uint32_t meshes[16 * kLdsMeshes];
 	float red[WAVES][6];
 	float red2[WAVES];
@@ -878,8 +895,9 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 
 			// 2. look-back, one view per lane of wave 0
 			if (wave == 0 && lane < vg) {
+				const bool act = (active >> lane) & 1ull;
 				uint32_t total = 0;
-				if ((active >> lane) & 1ull) {
+				if (act) {
 #pragma unroll
 					for (int q = 0; q < WAVES; q++) total += sh.cnt[lane][q];
 				}
@@ -892,66 +910,52 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 					excl = lookBackSerial(chain, T.chain, epoch);
 					publish(chain + T.chain, packStatus(epoch, kStatePrefix, excl + total));
 				}
-				sh.excl[lane] = excl;
+				if (act) {
+					uint32_t b = T.segBase + excl;
+#pragma unroll
+					for (int q = 0; q < WAVES; q++) {
+						sh.base[lane][q] = b;
+						b += sh.cnt[lane][q];
+					}
+				}
 				if (T.last && T.k < J.numStateSets)
 					globalMut<uint32_t>(J.counts)[uint64_t(g0 + lane) * J.numStateSets + T.k] = excl + total;
 			}
 			__syncthreads();
 
-			// 3. compacted writes: each item's visible drawables go straight from
-			// registers into the wave's 128-command staging ring (at the rank of
-			// the lane among the visible ones), and the ring is stored 64
-			// commands at a time as five 256-byte runs
+			// 3. compacted writes: each visible drawable's lane stores its command
+			// at the wave's output slot plus its rank among the item's visible
+			// lanes; consecutive visible lanes write consecutive commands, so a
+			// store instruction covers one contiguous run and the L2 merges the
+			// runs of an item into whole lines
 			auto writeViews = [&](auto lds) {
 				for (uint64_t am = active; am != 0ull; am &= am - 1ull) {
 					const uint32_t v = uint32_t(__builtin_ctzll(am));
 					const uint32_t run = uniform(sh.cnt[v][wave]);
 					if (run == 0) continue;
-					uint32_t wavePos = sh.excl[v];
-					for (uint32_t q = 0; q < wave; q++) wavePos += sh.cnt[v][q];
 					const ViewC V = loadView(views, g0 + v);
 					const gmut<uint32_t> out = globalMut<uint32_t>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
-					uint32_t* stage = sh.stage[wave];
-					uint32_t pos = uniform(T.segBase + wavePos);  // output slot of stage[0]
-					uint32_t pend = 0;                            // staged, not yet stored
+					uint32_t pos = uniform(sh.base[v][wave]);
 #pragma unroll
 					for (int j = 0; j < ITEMS; j++) {
-						const uint64_t m = sh.mask[v][wave][j];
+						const uint64_t m = uniform64(sh.mask[v][wave][j]);
 						const uint32_t n = uint32_t(__builtin_popcountll(m));
 						if (n == 0) continue;
 						if ((m >> lane) & 1ull) {
 							uint32_t count, first, vofs;
 							meshCommand<decltype(lds)::value>(meshes, mi[j], sp[j].w, viewCoord(V.r2, sp[j]), V, count,
 							                                  first, vofs);
-							stageCommand(stage, pend + maskRank(m), count, first, vofs,
-							             T.start + wave * WAVE_ITEMS + j * 64 + lane);
-						}
-						pend += n;
-						if (pend >= 64u) {
-							waveSync();
-							flushCommands(out, stage, pos, 64u, J.commandStride, lane);
-							pos += 64u;
-							pend -= 64u;
-							// the overflow (fewer than 64 commands) moves to the front
-							uint32_t carry[5];
-#pragma unroll
-							for (uint32_t k = 0; k < 5; k++) {
-								const uint32_t d = k * 64u + lane;
-								carry[k] = d < 5u * pend ? stage[320u + d] : 0u;
+							const uint32_t slot = pos + maskRank(m);
+							if (slot < J.commandStride) {
+								const gmut<uint32_t> o = out + uint64_t(slot) * 5ull;
+								o[0] = count;
+								o[1] = 1u;
+								o[2] = first;
+								o[3] = vofs;
+								o[4] = T.start + wave * WAVE_ITEMS + j * 64 + lane;
 							}
-							waveSync();
-#pragma unroll
-							for (uint32_t k = 0; k < 5; k++) {
-								const uint32_t d = k * 64u + lane;
-								if (d < 5u * pend) stage[d] = carry[k];
-							}
-							waveSync();
 						}
-					}
-					if (pend > 0u) {
-						waveSync();
-						flushCommands(out, stage, pos, pend, J.commandStride, lane);
-						waveSync();
+						pos += n;
 					}
 				}
 			};
