@@ -277,6 +277,49 @@ __device__ __forceinline__ uint32_t classifyTile(float4 ts, const ViewC& v)
 	return in ? 2u : 0u;
 }
 
+// Fused-multiply-add estimate of the drawable test: the smallest of the four
+// inside margins (near, far, x sides, y sides; visible when all are >= 0),
+// each computed with FMAs instead of the separately rounded glm expression.
+// Its difference from the same margins taken on the glm-order values, plus the
+// rounding that decides the glm-order comparisons, is below
+// prefilterMargin(): when |planeMin| exceeds that bound its sign gives
+// exactly the result of sphereVisible, else the caller evaluates
+// sphereVisible.  Inputs: finite centres, radius finite >= 0 or -inf (then
+// the centre is 0 and the result -inf, never visible); w0..w2 are the view's
+// translation column (r0.w, r1.w, r2.w), passed separately so that a caller
+// can keep them in vector registers (an FMA takes one scalar operand).
+__device__ __forceinline__ float planeMin(float4 s, const ViewC& v, float w0, float w1, float w2)
+{
+	const float vx = __builtin_fmaf(v.r0.x, s.x, __builtin_fmaf(v.r0.y, s.y, __builtin_fmaf(v.r0.z, s.z, w0)));
+	const float vy = __builtin_fmaf(v.r1.x, s.x, __builtin_fmaf(v.r1.y, s.y, __builtin_fmaf(v.r1.z, s.z, w1)));
+	const float vz = __builtin_fmaf(v.r2.x, s.x, __builtin_fmaf(v.r2.y, s.y, __builtin_fmaf(v.r2.z, s.z, w2)));
+	const float pn = (vz + s.w) - v.nearZ;
+	const float pf = (v.farZ - vz) + s.w;
+	const float px = __builtin_fmaf(-fabsf(vx), v.sx0, __builtin_fmaf(vz, v.sx1, s.w));
+	const float py = __builtin_fmaf(-fabsf(vy), v.sy0, __builtin_fmaf(vz, v.sy1, s.w));
+	return fminf(fminf(pn, pf), fminf(px, py));
+}
+
+// Bound for planeMin over spheres with |centre component| <= mc and radius
+// <= w.  With u = 2^-24 and S_r = (|r.x|+|r.y|+|r.z|)*mc + |r.w| for view row
+// r: each view coordinate differs between the FMA and the glm order by at
+// most 6u*S_r; the FMA margins add at most 2u of their operands, and the glm
+// comparisons are decided up to 2u of theirs, so every margin is within
+// u*(10*(|side0|*S + |side1|*S_z) + 3*w + |near| + |far|) of a decisive one.
+// The bound taken is 32u times the larger quantity
+// ((1 + sum |side|) * (S_x+S_y+S_z) + w + |near| + |far|), plus 2^-100
+// against underflow; it is NaN or infinite when a view constant is, and the
+// caller uses the estimate only when the bound is at most 2^100 (no operand
+// then comes near overflow).
+__device__ __forceinline__ float prefilterMargin(const ViewC& v, float mc, float w)
+{
+	const float sx = (fabsf(v.r0.x) + fabsf(v.r0.y) + fabsf(v.r0.z)) * mc + fabsf(v.r0.w);
+	const float sy = (fabsf(v.r1.x) + fabsf(v.r1.y) + fabsf(v.r1.z)) * mc + fabsf(v.r1.w);
+	const float sz = (fabsf(v.r2.x) + fabsf(v.r2.y) + fabsf(v.r2.z)) * mc + fabsf(v.r2.w);
+	const float sides = 1.0f + fabsf(v.sx0) + fabsf(v.sx1) + fabsf(v.sy0) + fabsf(v.sy1);
+	return (sides * (sx + sy + sz) + w + fabsf(v.nearZ) + fabsf(v.farZ)) * 0x1p-19f + 0x1p-100f;
+}
+
 __device__ __forceinline__ uint32_t laneId() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 __device__ __forceinline__ uint32_t maskRank(uint64_t m)
@@ -285,15 +328,6 @@ __device__ __forceinline__ uint32_t maskRank(uint64_t m)
 }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-
-// A 64-bit value that every lane holds alike (an LDS word all lanes read),
-// moved to scalar registers so that tests on it are scalar branches.
-__device__ __forceinline__ uint64_t uniform64(uint64_t x)
-{
-	const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
-	const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
-	return (uint64_t(hi) << 32) | uint64_t(lo);
-}
 
 __device__ __forceinline__ void waveSync()
 {
@@ -720,7 +754,6 @@ struct CullShared {
 	static constexpr int WAVES = BLOCK / 64;
 	static constexpr int TILE = BLOCK * ITEMS;
 
-	uint64_t mask[kViewGroup][WAVES][ITEMS];
 	uint32_t cnt[kViewGroup][WAVES];
 	uint32_t base[kViewGroup][WAVES];  // a wave's first output slot in its view's StateSet segment
 
@@ -783,10 +816,9 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 				s = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
 				m = 0;
 			} else if (m >= J.numMeshes) {
-				s.w = -__builtin_inff();  // invalid mesh: never visible
+				s = make_float4(0.f, 0.f, 0.f, -__builtin_inff());  // invalid mesh: never visible
 				m = 0;
 			}
-			sp[j] = s;
 			sp[j] = s;
 			mi[j] = m;
 
@@ -822,6 +854,7 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		__syncthreads();
 		bool tileUsable = false;
 		float4 tileSphere = make_float4(0.f, 0.f, 0.f, -1.f);
+		float tileCoord = 0.f;  // bound on |centre component| over the tile
 		if (useTileSphere) {
 			uint32_t anyBad = 0;
 #pragma unroll
@@ -856,6 +889,8 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			// inflate against rounding of the centre and radius above
 			const float slack = (fabsf(cx) + fabsf(cy) + fabsf(cz) + R) * (1.0f / 65536.0f);
 			tileSphere = make_float4(cx, cy, cz, R + slack);
+			tileCoord = fmaxf(fmaxf(fmaxf(fabsf(mn[0]), fabsf(mx[0])), fmaxf(fabsf(mn[1]), fabsf(mx[1]))),
+			                  fmaxf(fabsf(mn[2]), fabsf(mx[2])));
 		}
 		const bool emptyTile = T.count == 0;
 
@@ -870,24 +905,55 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			// the views that see part of the tile (every wave computes the same
 			// classes, so the mask is the same in every wave); the others
 			// contribute nothing and are skipped by all three phases
+			// (and the bound of the FMA estimate for the view; NaN: not used)
 			uint32_t cls = 0;
-			if (emptyTile) cls = 1;
-			else if (tileUsable && lane < vg) cls = classifyTile(tileSphere, loadView(views, g0 + lane));
+			float margin = __builtin_nanf("");
+			if (emptyTile) {
+				cls = 1;
+			} else if (tileUsable && lane < vg) {
+				const ViewC Vl = loadView(views, g0 + lane);
+				cls = classifyTile(tileSphere, Vl);
+				margin = prefilterMargin(Vl, tileCoord, tileSphere.w);
+			}
 			const uint64_t active = __ballot(lane < vg && cls != 1u);
 
-			// 1. visibility bits and per-wave counts
+			// 1. visibility bits and per-wave counts.  Lanes past the tile and
+			// invalid meshes hold the empty sphere, which no test passes.  The
+			// wave's item-j bits for view v are kept in lane v of mlo[j]/mhi[j].
+			uint32_t mlo[ITEMS], mhi[ITEMS];
+#pragma unroll
+			for (int j = 0; j < ITEMS; j++) mlo[j] = mhi[j] = 0u;
 			for (uint64_t am = active; am != 0ull; am &= am - 1ull) {
 				const uint32_t v = uint32_t(__builtin_ctzll(am));
 				const uint32_t c = uniform(__builtin_amdgcn_readlane(cls, v));
+				const float mg = __uint_as_float(uniform(__builtin_amdgcn_readlane(__float_as_uint(margin), v)));
 				const ViewC V = loadView(views, g0 + v);
+				uint64_t m[ITEMS];
+				if (c == 2u) {
+#pragma unroll
+					for (int j = 0; j < ITEMS; j++) m[j] = __ballot(sp[j].w >= 0.f);
+				} else if (mg <= 0x1p100f) {
+					// the translation column in vector registers once per view
+					float w0 = V.r0.w, w1 = V.r1.w, w2 = V.r2.w;
+					asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2));
+#pragma unroll
+					for (int j = 0; j < ITEMS; j++) {
+						const float p = planeMin(sp[j], V, w0, w1, w2);
+						m[j] = __ballot(p > 0.f);
+						const uint64_t amb = __ballot(!(fabsf(p) > mg));
+						if (amb != 0ull) m[j] = (m[j] & ~amb) | (__ballot(sphereVisible(sp[j], V)) & amb);
+					}
+				} else {
+#pragma unroll
+					for (int j = 0; j < ITEMS; j++) m[j] = __ballot(sphereVisible(sp[j], V));
+				}
 				uint32_t n = 0;
+				const bool mine = lane == v;
 #pragma unroll
 				for (int j = 0; j < ITEMS; j++) {
-					const uint32_t e = wave * WAVE_ITEMS + j * 64 + lane;
-					const bool vis = (e < T.count) && (c == 2u ? sp[j].w >= 0.f : sphereVisible(sp[j], V));
-					const uint64_t m = __ballot(vis);
-					if (lane == 0) sh.mask[v][wave][j] = m;
-					n += uint32_t(__builtin_popcountll(m));
+					mlo[j] = mine ? uint32_t(m[j]) : mlo[j];
+					mhi[j] = mine ? uint32_t(m[j] >> 32) : mhi[j];
+					n += uint32_t(__builtin_popcountll(m[j]));
 				}
 				if (lane == 0) sh.cnt[v][wave] = n;
 			}
@@ -938,7 +1004,8 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 					uint32_t pos = uniform(sh.base[v][wave]);
 #pragma unroll
 					for (int j = 0; j < ITEMS; j++) {
-						const uint64_t m = uniform64(sh.mask[v][wave][j]);
+						const uint64_t m = (uint64_t(uint32_t(__builtin_amdgcn_readlane(mhi[j], v))) << 32) |
+						                   uint64_t(uint32_t(__builtin_amdgcn_readlane(mlo[j], v)));
 						const uint32_t n = uint32_t(__builtin_popcountll(m));
 						if (n == 0) continue;
 						if ((m >> lane) & 1ull) {

@@ -197,6 +197,78 @@ def test_cull_edge_cases(gpu):
         ds.close()
 
 
+def _boundary_spheres(views, reps, rng):
+    """Spheres that touch one frustum plane of one view to within a few ulps:
+    for drawable i, view k = i % V and plane p = (i // V) % 4 (near, far, x
+    side, y side); the radius is the one that puts the glm-order comparison
+    of that plane exactly on its threshold, moved by -4..4 ulps.  Centres lie
+    inside the other planes where the plane allows it."""
+    f32 = np.float32
+    V = views.shape[0]
+    n = V * 4 * 9 * reps
+    out = np.zeros((n, 4), f32)
+    for i in range(n):
+        k, p, step = i % V, (i // V) % 4, (i // (4 * V)) % 9 - 4
+        v = views[k]
+        rows = v["viewRow"].astype(np.float64)
+        near, far = float(v["nearZ"]), float(v["farZ"])
+        tx = float(v["sideX"][1]) / float(v["sideX"][0])   # tan of the half angles
+        ty = float(v["sideY"][1]) / float(v["sideY"][0])
+        if p == 0:
+            qz = near * rng.uniform(0.2, 0.95)
+        elif p == 1:
+            qz = far * rng.uniform(1.01, 1.2)
+        else:
+            qz = rng.uniform(2 * near, 0.5 * far)
+        qx = rng.uniform(-0.9, 0.9) * tx * qz
+        qy = rng.uniform(-0.9, 0.9) * ty * qz
+        if p == 2:
+            qx = rng.choice([-1, 1]) * rng.uniform(1.05, 1.5) * tx * qz
+        if p == 3:
+            qy = rng.choice([-1, 1]) * rng.uniform(1.05, 1.5) * ty * qz
+        c = (rows[:, :3].T @ (np.array([qx, qy, qz]) - rows[:, 3])).astype(f32)
+        r = v["viewRow"]
+        vc = [(r[a][0] * c[0] + r[a][1] * c[1]) + (r[a][2] * c[2] + r[a][3]) for a in range(3)]
+        if p == 0:
+            w = f32(v["nearZ"] - vc[2])
+        elif p == 1:
+            w = f32(vc[2] - v["farZ"])
+        elif p == 2:
+            w = f32(abs(vc[0]) * v["sideX"][0]) - f32(vc[2] * v["sideX"][1])
+        else:
+            w = f32(abs(vc[1]) * v["sideY"][0]) - f32(vc[2] * v["sideY"][1])
+        w = f32(w)
+        for _ in range(abs(step)):
+            w = np.nextafter(w, f32(np.inf) if step > 0 else f32(0))
+        out[i, :3], out[i, 3] = c, max(w, f32(0))
+    return out
+
+
+def test_cull_boundary_spheres(gpu):
+    """Spheres within a few ulps of a plane: the batched kernel's FMA estimate
+    must defer to the glm-order test there, so results stay bit-exact."""
+    from cadr_amd.device import synchronize
+    rng = np.random.default_rng(11)
+    cams = scenes.random_cameras(scenes.SEED_BASE + 31, 20, 0, 8, box=100.0)
+    views = cull.make_views(cams)
+    s = _boundary_spheres(views, 40, rng)
+    n = s.shape[0]
+    meshes = scenes.make_meshes(91, 16, 3)
+    mi = rng.integers(0, 16, n).astype(np.uint32)
+    counts_in = scenes.balanced_counts(n, 8)
+    sc = scenes.Scene("boundary", [(np.zeros((n, 16), np.float32), None)], mi, meshes, counts_in, cams)
+    ds = cull.DeviceScene(sc, keep_levels=False)
+    ds.upload_spheres(s)
+    exp_cmds, exp_counts = oracle.cull(s, mi, meshes, counts_in, views, cmd_dtype=N.CMD_DTYPE, nthreads=8)
+    assert 0 < exp_counts.sum() < n * views.shape[0]
+    for block, items in [(256, 2), (256, 4), (256, 8), (512, 4), (128, 16)]:
+        print(f"boundary {block}x{items}", flush=True)
+        cmds, counts = _run(ds, views, cull.make_config(block, items))
+        _compare(cmds, counts, exp_cmds, exp_counts, counts_in, f"boundary {block}x{items}")
+    synchronize()
+    ds.close()
+
+
 def test_cull_view_after_batch_shares_workspace(plain):
     """Single-view launches, a batched launch and single-view launches again on one
     workspace: each launch leaves the ticket reset and the epoch advanced."""
