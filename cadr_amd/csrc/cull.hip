@@ -918,20 +918,21 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			const uint64_t active = __ballot(lane < vg && cls != 1u);
 
 			// 1. visibility bits and per-wave counts.  Lanes past the tile and
-			// invalid meshes hold the empty sphere, which no test passes.  The
-			// wave's item-j bits for view v are kept in lane v of mlo[j]/mhi[j].
-			uint32_t mlo[ITEMS], mhi[ITEMS];
+			// invalid meshes hold the empty sphere, which no test passes.  Each
+			// lane keeps its own bits: bit v%32 of vlo[j] (views 0-31 of the
+			// group) or vhi[j] (views 32-63) is item j's visibility in view v.
+			uint32_t vlo[ITEMS], vhi[ITEMS];
 #pragma unroll
-			for (int j = 0; j < ITEMS; j++) mlo[j] = mhi[j] = 0u;
+			for (int j = 0; j < ITEMS; j++) vlo[j] = vhi[j] = 0u;
 			for (uint64_t am = active; am != 0ull; am &= am - 1ull) {
 				const uint32_t v = uint32_t(__builtin_ctzll(am));
 				const uint32_t c = uniform(__builtin_amdgcn_readlane(cls, v));
 				const float mg = __uint_as_float(uniform(__builtin_amdgcn_readlane(__float_as_uint(margin), v)));
 				const ViewC V = loadView(views, g0 + v);
-				uint64_t m[ITEMS];
+				bool vis[ITEMS];
 				if (c == 2u) {
 #pragma unroll
-					for (int j = 0; j < ITEMS; j++) m[j] = __ballot(sp[j].w >= 0.f);
+					for (int j = 0; j < ITEMS; j++) vis[j] = sp[j].w >= 0.f;
 				} else if (mg <= 0x1p100f) {
 					// the translation column in vector registers once per view
 					float w0 = V.r0.w, w1 = V.r1.w, w2 = V.r2.w;
@@ -939,22 +940,25 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 #pragma unroll
 					for (int j = 0; j < ITEMS; j++) {
 						const float p = planeMin(sp[j], V, w0, w1, w2);
-						m[j] = __ballot(p > 0.f);
-						const uint64_t amb = __ballot(!(fabsf(p) > mg));
-						if (amb != 0ull) m[j] = (m[j] & ~amb) | (__ballot(sphereVisible(sp[j], V)) & amb);
+						vis[j] = p > 0.f;
+						const bool unsure = !(fabsf(p) > mg);
+						if (__ballot(unsure) != 0ull) vis[j] = unsure ? sphereVisible(sp[j], V) : vis[j];
 					}
 				} else {
 #pragma unroll
-					for (int j = 0; j < ITEMS; j++) m[j] = __ballot(sphereVisible(sp[j], V));
+					for (int j = 0; j < ITEMS; j++) vis[j] = sphereVisible(sp[j], V);
 				}
 				uint32_t n = 0;
-				const bool mine = lane == v;
+				const uint32_t bit = 1u << (v & 31u);
+				if (v < 32u) {
 #pragma unroll
-				for (int j = 0; j < ITEMS; j++) {
-					mlo[j] = mine ? uint32_t(m[j]) : mlo[j];
-					mhi[j] = mine ? uint32_t(m[j] >> 32) : mhi[j];
-					n += uint32_t(__builtin_popcountll(m[j]));
+					for (int j = 0; j < ITEMS; j++) vlo[j] |= vis[j] ? bit : 0u;
+				} else {
+#pragma unroll
+					for (int j = 0; j < ITEMS; j++) vhi[j] |= vis[j] ? bit : 0u;
 				}
+#pragma unroll
+				for (int j = 0; j < ITEMS; j++) n += uint32_t(__builtin_popcountll(__ballot(vis[j])));
 				if (lane == 0) sh.cnt[v][wave] = n;
 			}
 			__syncthreads();
@@ -993,38 +997,52 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			// at the wave's output slot plus its rank among the item's visible
 			// lanes; consecutive visible lanes write consecutive commands, so a
 			// store instruction covers one contiguous run and the L2 merges the
-			// runs of an item into whole lines
-			auto writeViews = [&](auto lds) {
-				for (uint64_t am = active; am != 0ull; am &= am - 1ull) {
-					const uint32_t v = uint32_t(__builtin_ctzll(am));
+			// runs of an item into whole lines.  Views 0-31 and 32-63 of the
+			// group take their bits from vlo and vhi.
+			auto writeHalf = [&](auto lds, uint64_t am, uint32_t v0,
+			                     const uint32_t (&bits)[ITEMS]) __attribute__((always_inline)) {
+				for (; am != 0ull; am &= am - 1ull) {
+					const uint32_t v = v0 + uint32_t(__builtin_ctzll(am));
 					const uint32_t run = uniform(sh.cnt[v][wave]);
 					if (run == 0) continue;
 					const ViewC V = loadView(views, g0 + v);
 					const gmut<uint32_t> out = globalMut<uint32_t>(J.commands) + uint64_t(g0 + v) * J.commandStride * 5ull;
-					uint32_t pos = uniform(sh.base[v][wave]);
+					const uint32_t base = uniform(sh.base[v][wave]);
+					// slot bounds per lane only when the wave's run can reach the stride
+					auto items = [&](auto checked) __attribute__((always_inline)) {
+						uint32_t pos = base;
 #pragma unroll
-					for (int j = 0; j < ITEMS; j++) {
-						const uint64_t m = (uint64_t(uint32_t(__builtin_amdgcn_readlane(mhi[j], v))) << 32) |
-						                   uint64_t(uint32_t(__builtin_amdgcn_readlane(mlo[j], v)));
-						const uint32_t n = uint32_t(__builtin_popcountll(m));
-						if (n == 0) continue;
-						if ((m >> lane) & 1ull) {
-							uint32_t count, first, vofs;
-							meshCommand<decltype(lds)::value>(meshes, mi[j], sp[j].w, viewCoord(V.r2, sp[j]), V, count,
-							                                  first, vofs);
-							const uint32_t slot = pos + maskRank(m);
-							if (slot < J.commandStride) {
-								const gmut<uint32_t> o = out + uint64_t(slot) * 5ull;
-								o[0] = count;
-								o[1] = 1u;
-								o[2] = first;
-								o[3] = vofs;
-								o[4] = T.start + wave * WAVE_ITEMS + j * 64 + lane;
+						for (int j = 0; j < ITEMS; j++) {
+							const bool vis = (bits[j] >> (v & 31u)) & 1u;
+							const uint64_t m = __ballot(vis);
+							const uint32_t n = uint32_t(__builtin_popcountll(m));
+							if (n == 0) continue;
+							if (vis) {
+								uint32_t count, first, vofs;
+								meshCommand<decltype(lds)::value>(meshes, mi[j], sp[j].w, viewCoord(V.r2, sp[j]), V,
+								                                  count, first, vofs);
+								const uint32_t r = maskRank(m);
+								if (!decltype(checked)::value || pos + r < J.commandStride) {
+									const gmut<uint32_t> o = out + uint64_t(pos) * 5ull + uint64_t(r * 5u);
+									o[0] = count;
+									o[1] = 1u;
+									o[2] = first;
+									o[3] = vofs;
+									o[4] = T.start + wave * WAVE_ITEMS + j * 64 + lane;
+								}
 							}
+							pos = uniform(pos + n);
 						}
-						pos += n;
-					}
+					};
+					if (base + run <= J.commandStride)
+						items(std::false_type{});
+					else
+						items(std::true_type{});
 				}
+			};
+			auto writeViews = [&](auto lds) __attribute__((always_inline)) {
+				writeHalf(lds, active & 0xffffffffull, 0u, vlo);
+				writeHalf(lds, active >> 32, 32u, vhi);
 			};
 			if (meshes.inLds)
 				writeViews(std::true_type{});
