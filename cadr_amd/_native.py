@@ -154,6 +154,7 @@ SIGNATURES = {
     "cadrEventQuery": (c_int, [c_void_p]),
     "cadrEventElapsedTime": (c_int, [ctypes.POINTER(c_float), c_void_p, c_void_p]),
     "cadrProcessDrawables": (c_int, [ctypes.POINTER(PushConstants), c_u32, c_u64, c_void_p]),
+    "cadrScatterDrawables": (c_int, [c_u64, c_u64, c_u64, c_u64, c_u32, c_void_p]),
     "cadrPropagateTransforms": (c_int, [c_void_p, c_void_p]),
     "cadrPropagateTransformsBatch": (c_int, [c_u64, c_void_p, c_u32, c_void_p]),
     "cadrComputeWorldSpheres": (c_int, [c_u64, c_u64, c_u64, c_u32, c_u64, c_u64, c_void_p]),

@@ -103,9 +103,39 @@ __global__ __launch_bounds__(256) void processDrawablesKernel(const uint64_t* __
 	                                         uint32_t(drawableDataPtr >> 32));
 }
 
+// Delta upload of the drawable list: list[slots[i]] = records[i], one lane
+// per record, each 48-byte record moved as three 16-byte loads and stores.
+__global__ __launch_bounds__(256) void scatterDrawablesKernel(const uint4* __restrict__ records,
+                                                              const uint32_t* __restrict__ slots, uint32_t count,
+                                                              uint4* __restrict__ list, uint64_t listSlots)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if (i >= count) return;
+	const uint64_t slot = slots[i];
+	if (slot >= listSlots) return;
+	const uint4 a = records[3ull * i + 0], b = records[3ull * i + 1], c = records[3ull * i + 2];
+	list[3 * slot + 0] = a;
+	list[3 * slot + 1] = b;
+	list[3 * slot + 2] = c;
+}
+
 }  // namespace cadr
 
 using namespace cadr;
+
+CADR_API int cadrScatterDrawables(uint64_t drawableList, uint64_t listSlots, uint64_t records, uint64_t slots,
+                                  uint32_t count, cadrStream stream)
+{
+	if (count == 0) return ok();
+	CADR_REQUIRE(drawableList && records && slots, "cadrScatterDrawables: null device address");
+	CADR_REQUIRE((drawableList % 16) == 0 && (records % 16) == 0 && (slots % 4) == 0,
+	             "cadrScatterDrawables: drawableList and records must be 16-byte aligned, slots 4-byte aligned");
+	hipLaunchKernelGGL(scatterDrawablesKernel, dim3((count + 255u) / 256u), dim3(256), 0, hipStream_t(stream),
+	                   reinterpret_cast<const uint4*>(records), reinterpret_cast<const uint32_t*>(slots), count,
+	                   reinterpret_cast<uint4*>(drawableList), listSlots);
+	CADR_CHECK_HIP(hipGetLastError(), "scatterDrawablesKernel launch");
+	return ok();
+}
 
 CADR_API int cadrProcessDrawables(const cadrProcessDrawablesPushConstants* pc, uint32_t handleLevel,
                                   uint64_t numDrawables, cadrStream stream)

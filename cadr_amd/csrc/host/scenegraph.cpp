@@ -98,6 +98,7 @@ void Drawable::createInternal(Geometry& g, uint32_t off, MatrixList& ml, DataAll
 	if (_indexIntoStateSet != ~0u) {
 		if (_stateSet == &ss) {
 			_stateSet->_drawableDataList[_indexIntoStateSet] = gpuDataOf(g, off, ml, nullptr);
+			_stateSet->markDirty(_indexIntoStateSet);
 			return;
 		}
 		_stateSet->removeDrawableInternal(*this);
@@ -142,12 +143,29 @@ void StateSet::removeChild(StateSet& child)
 	p.erase(std::find(p.begin(), p.end(), this));
 }
 
+void StateSet::markDirty(uint32_t index) noexcept
+{
+	if (_allDirty) return;
+	if (_dirty.size() >= std::max<size_t>(64, _drawableDataList.size() / 4)) {
+		_allDirty = true;
+		_dirty.clear();
+		return;
+	}
+	try {
+		_dirty.push_back(index);
+	} catch (...) {
+		_allDirty = true;
+		_dirty.clear();
+	}
+}
+
 void StateSet::appendDrawableInternal(Drawable& d, const DrawableGpuData& gpuData)
 {
 	d._stateSet = this;
 	d._indexIntoStateSet = uint32_t(_drawableDataList.size());
 	_drawableDataList.emplace_back(gpuData);
 	_drawablePtrList.emplace_back(&d);
+	markDirty(d._indexIntoStateSet);
 }
 
 // Removal moves the last drawable into the hole (StateSet.cpp:29-47), so the
@@ -161,6 +179,7 @@ void StateSet::removeDrawableInternal(Drawable& d) noexcept
 		Drawable* moved = _drawablePtrList[last];
 		_drawablePtrList[i] = moved;
 		moved->_indexIntoStateSet = i;
+		markDirty(i);
 	}
 	_drawableDataList.pop_back();
 	_drawablePtrList.pop_back();

@@ -152,6 +152,7 @@ public:
 private:
 	void appendDrawableInternal(Drawable& d, const DrawableGpuData& gpuData);
 	void removeDrawableInternal(Drawable& d) noexcept;
+	void markDirty(uint32_t index) noexcept;
 
 	Renderer* _renderer;
 	bool _skipRecording = true;
@@ -160,7 +161,15 @@ private:
 	std::vector<Drawable*> _drawablePtrList;
 	std::vector<StateSet*> _children;
 	std::vector<StateSet*> _parents;
+	// records changed since the renderer last uploaded this StateSet: every
+	// index whose record differs from the uploaded one at the same index is
+	// listed (append, removal's move into the hole, in-place create), or
+	// _allDirty is set (a new StateSet, or more changes than are worth
+	// listing)
+	std::vector<uint32_t> _dirty;
+	bool _allDirty = true;
 	friend class Drawable;
+	friend class Renderer;
 };
 
 }  // namespace CadR

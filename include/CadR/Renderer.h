@@ -17,6 +17,19 @@
 // recording the draws (StateSet.cpp:233-237) happens before the copy here;
 // recordSceneRendering then only reports the per-StateSet draw ranges.
 //
+// Resident drawable records (on by default on a device): the drawable list
+// stays in device memory between frames.  When the StateSet ranges are the
+// frame before's, only the records that changed since (append, removal's
+// move into the hole, in-place Drawable::create) are sent, as (slot, record)
+// pairs applied by cadrScatterDrawables; when ranges moved, the list is
+// rebuilt in a second device buffer by device-to-device copies of every
+// StateSet it already holds plus those deltas, and StateSets new to the list
+// are copied whole.  The list the gather reads is then the one the full copy
+// would give.  setResidentDrawables(false) restores the reference's full
+// staging memcpy and copy each frame (StateSet.cpp:233-237,
+// Renderer.cpp:635-644).  drawableBufferAddress() can change from frame to
+// frame in resident mode.
+//
 // FrameInfo carries the reference's fields (FrameInfo.h:15-32); GPU
 // timestamps are HIP events converted to nanoseconds from the frame's first
 // GPU event.
@@ -91,6 +104,12 @@ public:
 	uint64_t drawablePointersBufferAddress() const { return _pointersBuffer; }
 	size_t bufferCapacity() const { return _capacity; }
 
+	// resident drawable records (see above) and the host -> device bytes the
+	// last recordDrawableProcessing sent for the drawable list
+	void setResidentDrawables(bool on);
+	bool residentDrawables() const { return _resident; }
+	size_t lastDrawableUploadBytes() const { return _lastUploadBytes; }
+
 	// frame timing (Renderer.cpp:1025-1066)
 	void setCollectFrameInfo(bool on) { _collectFrameInfo = on; }
 	bool collectFrameInfo() const { return _collectFrameInfo; }
@@ -102,6 +121,9 @@ public:
 private:
 	void releaseBuffers() noexcept;
 	void recordEvent(int index, void* stream);
+	void uploadFullList(void* stream, size_t numDrawables);
+	void uploadResidentList(void* stream, const std::vector<DrawRange>& ranges);
+	void reserveDelta(size_t count);
 
 	static Renderer* _defaultRenderer;
 	int _device = -1;
@@ -117,6 +139,20 @@ private:
 	void* _events[FrameInfo::gpuTimestampPoolSize] = {};
 	bool _eventRecorded[FrameInfo::gpuTimestampPoolSize] = {};
 	size_t _infoFrame = size_t(-1);
+
+	// resident drawable list
+	bool _resident = true;
+	bool _residentValid = false;             // _drawableBuffer holds _residentRanges' records
+	uint64_t _drawableBufferSpare = 0;       // rebuild target when ranges move
+	std::vector<DrawRange> _residentRanges;  // layout of _drawableBuffer
+	uint8_t* _delta = nullptr;               // pinned: records (48 B each) then slots (uint32)
+	uint64_t _deltaDevice = 0;               // device copy of _delta
+	size_t _deltaCapacity = 0;               // records _delta holds
+	void* _uploadDone = nullptr;             // host staging may be rewritten once this completes
+	bool _uploadPending = false;
+	void* _listFree = nullptr;               // the last gather has read the drawable list
+	bool _listFreePending = false;
+	size_t _lastUploadBytes = 0;
 };
 
 }  // namespace CadR

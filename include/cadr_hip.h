@@ -180,6 +180,17 @@ typedef struct cadrProcessDrawablesPushConstants {
 int cadrProcessDrawables(const cadrProcessDrawablesPushConstants* pushConstants,
                          uint32_t handleLevel, uint64_t numDrawables, cadrStream stream);
 
+/* Delta update of a device-resident drawable list (48-byte DrawableGpuData
+ * records, listSlots of them): drawableList[slots[i]] = records[i] for
+ * i < count; slots at or past listSlots are skipped.  records (16-byte
+ * aligned) and slots (uint32) are device arrays.  It replaces the per-frame
+ * copy of the whole staged list (StateSet.cpp:233-237 memcpy into staging,
+ * Renderer.cpp:635-644 staging -> device copy) by the records that changed;
+ * the facade's Renderer uses it when the drawable order is unchanged.  The
+ * order of writes to one slot listed twice is unspecified. */
+int cadrScatterDrawables(uint64_t drawableList, uint64_t listSlots, uint64_t records, uint64_t slots,
+                         uint32_t count, cadrStream stream);
+
 /* ------------------------------------------------------------------------ */
 /* Transform propagation (one tree level per call) and world bounding
  * spheres.  Matrices are column-major float mat4 (64 B), as glm::mat4 and

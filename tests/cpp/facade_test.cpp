@@ -7,7 +7,11 @@
 //   facade_test gpu   Renderer on device 0: full frames through
 //                     prepareSceneRendering / recordDrawableProcessing at
 //                     handle levels 1, 2 and 3, outputs checked against the
-//                     handle -> address bookkeeping of the host side
+//                     handle -> address bookkeeping of the host side; then
+//                     frames with scene edits between them (appends,
+//                     swap-removals, in-place and cross-StateSet
+//                     Drawable::create, StateSet edges), the resident
+//                     drawable list checked the same way every frame
 //
 // Like the reference's tests it prints nothing on success but "OK" and exits
 // non-zero with a message on the first failure.
@@ -314,6 +318,140 @@ static void gpuFrame(int handleLevel, size_t numDrawables, uint32_t seed)
 	(void)hipStreamDestroy(stream);
 }
 
+// Frames with edits between them on one renderer: every frame's outputs
+// against the host bookkeeping, and the resident list's uploads against what
+// the edits touched.
+static void gpuResidentFrames(uint32_t seed)
+{
+	std::printf("gpu resident frames: seed %u\n", seed);
+	std::fflush(stdout);
+	Renderer r(0, false);
+	DataStorage& ds = r.dataStorage();
+	std::mt19937 rng(seed);
+	for (uint64_t i = 0; i < 3000; i++) ds.createHandle();  // handle level 2
+
+	Scene sc;
+	const int G = 5;
+	std::vector<std::vector<PrimitiveSet>> psData;
+	for (int g = 0; g < G; g++) {
+		sc.geometries.push_back(std::make_unique<Geometry>(r));
+		std::vector<uint8_t> v(64 + 32 * g), idx(48 + 16 * g);
+		sc.geometries.back()->uploadVertexData(v.data(), v.size());
+		sc.geometries.back()->uploadIndexData(idx.data(), idx.size());
+		std::vector<PrimitiveSet> ps(3);
+		for (auto& p : ps) p = {uint32_t(rng() % 100000), uint32_t(rng())};
+		sc.geometries.back()->uploadPrimitiveSetData(ps.data(), ps.size() * sizeof(PrimitiveSet));
+		psData.push_back(ps);
+	}
+	// root -> 1, 2; 1 -> 3; 2 -> 3 (StateSet 3 drawn twice); 4 and 5 attached later
+	const int K = 6;
+	for (int k = 0; k < K; k++) sc.stateSets.push_back(std::make_unique<StateSet>(r));
+	StateSet& root = *sc.stateSets[0];
+	root.appendChild(*sc.stateSets[1]);
+	root.appendChild(*sc.stateSets[2]);
+	sc.stateSets[1]->appendChild(*sc.stateSets[3]);
+	sc.stateSets[2]->appendChild(*sc.stateSets[3]);
+	struct Truth {
+		int g, p, ml;
+	};
+	std::unordered_map<const Drawable*, Truth> truth;
+	const int M = 64;
+	for (int i = 0; i < M; i++) {
+		sc.matrixLists.push_back(std::make_unique<MatrixList>(r));
+		std::vector<float> m(16 * (1 + i % 3), 1.0f);
+		sc.matrixLists.back()->setMatrices(m.data(), 1 + i % 3);
+	}
+	auto addDrawable = [&](StateSet& ss) {
+		const int g = int(rng() % G), p = int(rng() % 3), ml = int(rng() % M);
+		sc.drawables.push_back(std::make_unique<Drawable>(*sc.geometries[g], uint32_t(8 * p), *sc.matrixLists[ml], ss));
+		truth[sc.drawables.back().get()] = {g, p, ml};
+	};
+	for (int i = 0; i < 20000; i++) addDrawable(*sc.stateSets[1 + rng() % 5]);
+
+	hipStream_t stream;
+	if (hipStreamCreate(&stream) != hipSuccess) throw std::runtime_error("hipStreamCreate");
+	for (int frame = 0; frame < 14; frame++) {
+		size_t touched = 0;  // records the edits of this frame changed (at most)
+		bool moved = false;  // the StateSet ranges may move
+		if (frame == 2) {
+			sc.stateSets[0]->appendChild(*sc.stateSets[4]);
+			moved = true;
+		}
+		if (frame == 5) {
+			sc.stateSets[1]->removeChild(*sc.stateSets[3]);
+			moved = true;
+		}
+		if (frame == 7) r.setResidentDrawables(false);
+		if (frame == 8) r.setResidentDrawables(true);
+		if (frame >= 1 && frame != 4) {
+			const int edits = frame % 3 == 0 ? 200 : 20;
+			for (int e = 0; e < edits; e++) {
+				Drawable& d = *sc.drawables[rng() % sc.drawables.size()];
+				const int kind = int(rng() % 4);
+				if (kind == 0 && d.isValid()) {
+					d.destroy();  // the last record of its StateSet moves into the hole
+					moved = true;
+				} else if (kind == 1) {
+					addDrawable(*sc.stateSets[1 + rng() % 5]);
+					moved = true;
+				} else if (kind == 2 && d.isValid()) {
+					const int g = int(rng() % G), p = int(rng() % 3), ml = int(rng() % M);
+					d.create(*sc.geometries[g], uint32_t(8 * p), *sc.matrixLists[ml], d.stateSet());  // in place
+					truth[&d] = {g, p, ml};
+				} else if (kind == 3 && d.isValid()) {
+					const int g = int(rng() % G), p = int(rng() % 3), ml = int(rng() % M);
+					d.create(*sc.geometries[g], uint32_t(8 * p), *sc.matrixLists[ml], *sc.stateSets[1 + rng() % 5]);
+					truth[&d] = {g, p, ml};
+					moved = true;
+				}
+				touched += 2;
+			}
+		}
+		r.beginFrame();
+		r.executeCopyOperations();
+		r.beginRecording(stream);
+		const size_t n = r.prepareSceneRendering(root);
+		r.recordDrawableProcessing(stream, n);
+		std::vector<DrawRange> ranges;
+		r.recordSceneRendering(stream, root, ranges);
+		r.endRecording(stream);
+		r.endFrame();
+		if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("hipStreamSynchronize");
+		const size_t sent = r.lastDrawableUploadBytes();
+		std::printf("  frame %d: %zu drawables, %zu bytes sent\n", frame, n, sent);
+		if (frame == 0 || frame == 7 || frame == 8) {
+			CHECK(sent == n * sizeof(DrawableGpuData), "full list sent");
+		} else if (frame != 2) {  // frame 2 attaches a StateSet the list does not hold yet
+			CHECK(sent <= touched * (sizeof(DrawableGpuData) + 4) * 2,
+			      std::string("only the changed records sent") + (moved ? " (ranges moved)" : ""));
+		}
+		if (frame == 4) CHECK(sent == 0, "no edits, nothing sent");
+		std::vector<cadrDrawIndirectCommand> ind(n);
+		std::vector<cadrDrawablePointers> ptr(n);
+		ds.provider().download(ind.data(), r.drawIndirectBufferAddress(), n * sizeof(cadrDrawIndirectCommand));
+		ds.provider().download(ptr.data(), r.drawablePointersBufferAddress(), n * sizeof(cadrDrawablePointers));
+		size_t slot = 0;
+		for (const DrawRange& range : ranges) {
+			for (size_t j = 0; j < range.count; j++, slot++) {
+				const Drawable& d = range.stateSet->getDrawable(j);
+				const Truth& t = truth.at(&d);
+				const Geometry& g = *sc.geometries[t.g];
+				const MatrixList& ml = *sc.matrixLists[t.ml];
+				const PrimitiveSet& ps = psData[t.g][t.p];
+				CHECK(ind[slot].vertexCount == ps.indexCount && ind[slot].firstVertex == ps.startIndex &&
+				          ind[slot].instanceCount == uint32_t(1 + t.ml % 3),
+				      "frame " + std::to_string(frame) + ": indirect command of slot " + std::to_string(slot));
+				CHECK(ptr[slot].vertexDataPtr == g.vertexDataAllocation().deviceAddress() &&
+				          ptr[slot].indexDataPtr == g.indexDataAllocation().deviceAddress() &&
+				          ptr[slot].matrixListPtr == ml.deviceAddress() && ptr[slot].drawableDataPtr == 0,
+				      "frame " + std::to_string(frame) + ": drawable pointers of slot " + std::to_string(slot));
+			}
+		}
+		CHECK(slot == n, "ranges cover every slot");
+	}
+	(void)hipStreamDestroy(stream);
+}
+
 int main(int argc, char** argv)
 {
 	const std::string mode = argc > 1 ? argv[1] : "cpu";
@@ -326,6 +464,8 @@ int main(int argc, char** argv)
 			gpuFrame(1, 500, 1);
 			gpuFrame(2, 20000, 2);
 			gpuFrame(3, 20000, 3);
+			gpuResidentFrames(4);
+			gpuResidentFrames(5);
 		} else {
 			std::fprintf(stderr, "usage: facade_test cpu|gpu\n");
 			return 2;
