@@ -15,7 +15,10 @@ Further legs, reported beside it:
   uneven   uneven64: 64 scenes (10k..5M drawables, depth 1..4, 1..32 views),
            every tree propagated and every view culled in one graph per frame;
   served   served128: 128 resident scenes under Poisson cull requests through
-           the serving loop, p50/p99/p999 latency and throughput per rate.
+           the serving loop, p50/p99/p999 latency and throughput per rate;
+  resident facade frames at 10M drawables with 0.1% of the drawables edited
+           per frame: the reference's full per-frame copy of the drawable
+           list against the resident list with delta uploads.
 Kernel shapes come from the autotuner (cadrCullAutotune) unless --no-tune.
 
 Prints one JSON line (metric / value / roofline / cpu_baseline ...).
@@ -75,6 +78,8 @@ def parse():
     p.add_argument("--no-uneven", action="store_true")
     p.add_argument("--no-served", action="store_true")
     p.add_argument("--no-gather", action="store_true")
+    p.add_argument("--no-resident", action="store_true")
+    p.add_argument("--resident-drawables", type=int, default=10_000_000)
     p.add_argument("--served-rates", default="5000,10000,20000,50000", help="Poisson arrival rates (req/s)")
     p.add_argument("--served-seconds", type=float, default=1.0, help="arrival window per rate")
     p.add_argument("--served-budget-gib", type=float, default=64.0, help="output memory budget of the server")
@@ -278,11 +283,37 @@ def main():
     if not args.no_served:
         result["served"] = bench_served(args, served_config, N, cull, scenes, device)
 
+    if not args.no_resident:
+        result["resident"] = bench_resident(args)
+
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, stream, lib, N, device)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)
+
+
+def bench_resident(args, frames=10):
+    """Facade frames (cadr_amd/bin/facade_frames, C++): the same scene and
+    edits with the full per-frame copy of the drawable list and with the
+    resident list; medians of host wall clock per frame."""
+    import subprocess
+    exe = os.path.join(ROOT, "cadr_amd", "bin", "facade_frames")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is not built (make -C cadr_amd/csrc)")
+    n = args.resident_drawables
+    edits = max(1, n // 1000)
+    p = subprocess.run([exe, str(n), str(frames), str(edits)], capture_output=True, text=True, timeout=900)
+    if p.returncode != 0:
+        raise RuntimeError(f"facade_frames failed ({p.returncode}): {p.stderr[-2000:]}")
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    r["workload"] = (f"facade frames: {n} drawables in 64 StateSets, {edits} drawables re-created in place per "
+                     f"frame (Drawable::create), gather at handle level {r['handle_level']}; frame = beginFrame .. "
+                     f"stream done, host wall clock, median of {frames}")
+    r["speedup"] = r["full_copy_ms"] / r["resident_ms"]
+    log(f"resident: full copy {r['full_copy_ms']:.3f} ms/frame ({r['full_copy_bytes']} B), resident "
+        f"{r['resident_ms']:.3f} ms/frame ({r['resident_bytes']} B), {r['speedup']:.1f}x")
+    return r
 
 
 def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
