@@ -21,8 +21,8 @@
 //
 // Three kernels:
 //   cullViewKernel - one view: one workgroup per tile, visibility kept in
-//     registers, a 64-lane parallel look-back in wave 0, commands staged in
-//     LDS and stored as whole runs.  It resets its ticket and advances the
+//     registers, a 64-lane parallel look-back in wave 0, each visible
+//     drawable's lane storing its command at its compacted slot.  It resets its ticket and advances the
 //     epoch itself (the last workgroup to finish), so a view costs one launch.
 //   cullFusedKernel - the same for one view, with the tile's spheres computed
 //     from the last transform level instead of read (cadrPropagateCullView).
@@ -456,40 +456,6 @@ __device__ __forceinline__ float4 meshSphere(const MeshTable& M, uint32_t m)
 	}
 }
 
-// A wave's compacted commands are staged in LDS (5 dwords each, a stride
-// that is conflict-free across the LDS banks) and then stored as runs of 64
-// consecutive dwords, so every store instruction writes 256 contiguous bytes
-// and HBM sees whole lines instead of partial ones.
-__device__ __forceinline__ void stageCommand(uint32_t* stage, uint32_t slot, uint32_t count, uint32_t first,
-                                             uint32_t vofs, uint32_t drawable)
-{
-	uint32_t* o = stage + slot * 5u;
-	o[0] = count;
-	o[1] = 1u;
-	o[2] = first;
-	o[3] = vofs;
-	o[4] = drawable;
-}
-
-// Stores n staged commands (n <= 64) to commands[first .. first+n), skipping
-// slots at or beyond the stride.
-__device__ __forceinline__ void flushCommands(gmut<uint32_t> out, const uint32_t* stage, uint32_t first, uint32_t n,
-                                              uint32_t stride, uint32_t lane)
-{
-	const gmut<uint32_t> dst = out + uint64_t(first) * 5ull;
-	if (n == 64u && first + 64u <= stride) {
-		// a full run inside the stride: no per-dword bounds, constant offsets
-#pragma unroll
-		for (uint32_t k = 0; k < 5; k++) __builtin_nontemporal_store(stage[k * 64u + lane], dst + (k * 64u + lane));
-		return;
-	}
-#pragma unroll
-	for (uint32_t k = 0; k < 5; k++) {
-		const uint32_t d = k * 64u + lane;
-		if (d < 5u * n && first + d / 5u < stride) __builtin_nontemporal_store(stage[d], dst + d);
-	}
-}
-
 // Mesh table of a job staged in LDS, field-major (see MeshTable), when it is
 // small; returns whether the LDS copy is in use.
 template <int BLOCK>
@@ -617,8 +583,7 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 template <int BLOCK, int ITEMS, bool LDS, bool FUSED>
 __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCullJob& J, uint32_t t, uint32_t epoch,
                                              const MeshTable& meshes, uint32_t (*sCnt)[ITEMS], uint32_t* sExclP,
-                                             uint32_t* sStage, float4* sMat, uint32_t tid, uint32_t lane,
-                                             uint32_t wave)
+                                             float4* sMat, uint32_t tid, uint32_t lane, uint32_t wave)
 {
 	constexpr int WAVES = BLOCK / 64, WAVE_ITEMS = 64 * ITEMS;
 	const TileDesc T = loadTile(J, t, BLOCK * ITEMS);
@@ -672,7 +637,6 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 		for (int j = 0; j < ITEMS; j++) pos += sCnt[q][j];
 	pos = uniform(pos);
 	const gmut<uint32_t> out = globalMut<uint32_t>(J.commands);
-	uint32_t* stage = sStage + wave * 320u;
 #pragma unroll
 	for (int j = 0; j < ITEMS; j++) {
 		const uint32_t n = uint32_t(__builtin_popcountll(mask[j]));
@@ -680,12 +644,17 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 		if ((mask[j] >> lane) & 1ull) {
 			uint32_t count, first, vofs;
 			meshCommand<LDS>(meshes, mi[j], sp[j].w, vz[j], V, count, first, vofs);
-			stageCommand(stage, maskRank(mask[j]), count, first, vofs, T.start + wave * WAVE_ITEMS + j * 64 + lane);
+			const uint32_t r = maskRank(mask[j]);
+			if (pos + r < J.commandStride) {
+				const gmut<uint32_t> o = out + uint64_t(pos) * 5ull + uint64_t(r * 5u);
+				o[0] = count;
+				o[1] = 1u;
+				o[2] = first;
+				o[3] = vofs;
+				o[4] = T.start + wave * WAVE_ITEMS + j * 64 + lane;
+			}
 		}
-		waveSync();
-		flushCommands(out, stage, pos, n, J.commandStride, lane);
-		waveSync();
-		pos += n;
+		pos = uniform(pos + n);
 	}
 }
 
@@ -699,7 +668,6 @@ __device__ __forceinline__ void cullViewBody(const CullParams& P)
 	__shared__ uint32_t sCnt[WAVES][ITEMS];
 	__shared__ uint32_t sExcl;
 	__shared__ uint32_t sMeshes[16 * kLdsMeshes];
-	__shared__ uint32_t sStage[WAVES * 320];
 
 	const uint32_t tid = threadIdx.x, lane = laneId(), wave = tid / 64;
 	const uint32_t epoch = __hip_atomic_load(&P.ctl[kCtlEpoch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -712,10 +680,9 @@ __device__ __forceinline__ void cullViewBody(const CullParams& P)
 	if (t < P.totalWork) {
 		float4* mat = &sMat[FUSED ? wave : 0][0];
 		if (ldsMeshes)
-			cullViewTile<BLOCK, ITEMS, true, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, mat, tid, lane, wave);
+			cullViewTile<BLOCK, ITEMS, true, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, mat, tid, lane, wave);
 		else
-			cullViewTile<BLOCK, ITEMS, false, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, sStage, mat, tid, lane,
-			                                          wave);
+			cullViewTile<BLOCK, ITEMS, false, FUSED>(P, J, t, epoch, meshes, sCnt, &sExcl, mat, tid, lane, wave);
 	}
 
 	// the last workgroup to finish resets the ticket and starts a new epoch
