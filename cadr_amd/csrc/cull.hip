@@ -765,7 +765,7 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		const TileDesc T = loadTile(J, t, S::TILE);
 		const bool ldsMeshes = stageMeshes<BLOCK>(sh.meshes, J, tid);
 
-		// load the tile: wave-contiguous runs, staged in LDS, bounds for the tile sphere
+		// load the tile into registers (wave-contiguous runs) and bound it for the tile sphere
 		const gptr<float4> spheres = globalPtr<float4>(J.spheres);
 		const gptr<uint32_t> meshIndex = globalPtr<uint32_t>(J.meshIndex);
 		float4 sp[ITEMS];
