@@ -88,6 +88,35 @@ def test_gpu_gather_zero_and_errors(gpu):
 
 
 @pytest.mark.gpu
+def test_gpu_scatter_drawables(gpu):
+    """cadrScatterDrawables: list[slots[i]] = records[i]; slots past the list
+    are skipped; untouched slots keep their bytes; count 0 is a no-op."""
+    from cadr_amd.device import DeviceBuffer, synchronize
+    lib = N.load()
+    rng = np.random.default_rng(21)
+    slots_total = 70_000
+    before = rng.integers(0, 2**32, (slots_total, 12), dtype=np.uint64).astype(np.uint32)
+    lst = DeviceBuffer.from_array(before)
+    count = 20_000
+    slots = rng.choice(slots_total, count - 3, replace=False).astype(np.uint32)
+    slots = np.concatenate([slots, np.array([slots_total, slots_total + 5, 0xFFFFFFFF], np.uint32)])
+    records = rng.integers(0, 2**32, (count, 12), dtype=np.uint64).astype(np.uint32)
+    dr, ds_ = DeviceBuffer.from_array(records), DeviceBuffer.from_array(slots)
+    assert lib.cadrScatterDrawables(lst.address, slots_total, dr.address, ds_.address, 0, None) == N.SUCCESS
+    N.check(lib.cadrScatterDrawables(lst.address, slots_total, dr.address, ds_.address, count, None))
+    synchronize()
+    got = lst.download(np.uint32).reshape(slots_total, 12)
+    exp = before.copy()
+    exp[slots[:-3]] = records[:-3]
+    assert np.array_equal(got, exp)
+    assert lib.cadrScatterDrawables(0, slots_total, dr.address, ds_.address, 5, None) == N.ERROR_INVALID_VALUE
+    assert lib.cadrScatterDrawables(lst.address + 8, slots_total, dr.address, ds_.address, 5, None) == \
+        N.ERROR_INVALID_VALUE
+    for b in (lst, dr, ds_):
+        b.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,first_handle", [(300_000, 1), (2_000_000, 1 << 22), (1_000_000, (1 << 22) - 5000)])
 def test_gpu_gather_bulk_scene(gpu, n, first_handle):
     """Large consecutive-handle scenes (the reference's numbering) at levels 2 and
