@@ -105,33 +105,39 @@ DataStorage::~DataStorage() noexcept
 // and more, 16 B otherwise (CircularAllocationMemory.h:565-568).
 static size_t alignmentFor(size_t numBytes) { return numBytes >= 64 ? 64 : 16; }
 
+// Block choice of DataStorage::allocInternal (DataStorage.cpp:31-101): the
+// first allocation block is tried, then the second; when neither has room, a
+// new large block becomes the second and the old second the first.  The
+// first block is 64 KiB (or larger for a large first allocation), the second
+// 2 MiB, later ones 32 MiB (Renderer.h:104-106).  A block whose allocations
+// are all freed is reused from its start, so an emptied storage places new
+// allocations from the start of its first block again.
 uint32_t DataStorage::blockFor(size_t numBytes, size_t align)
 {
-	// newest blocks first: they are the ones with free space at the top
-	for (size_t i = _blocks.size(); i-- > 0;) {
-		Block& b = _blocks[i];
+	auto fits = [&](uint32_t i) {
+		const Block& b = _blocks[i];
 		const size_t at = (b.top + align - 1) / align * align;
-		if (at + numBytes <= b.size) return uint32_t(i);
-		if (_blocks.size() - i > 2) break;   // only the two most recent blocks take new allocations
-	}
-	// block sizes of DataStorage::allocInternal (DataStorage.cpp:31-101)
-	size_t size = _blocks.empty() ? (numBytes < smallMemorySize ? smallMemorySize
-	                                                            : numBytes < mediumMemorySize ? mediumMemorySize
-	                                                                                          : std::max(numBytes, largeMemorySize))
-	              : _blocks.size() == 1 ? (numBytes < mediumMemorySize ? mediumMemorySize : std::max(numBytes, largeMemorySize))
-	                                    : std::max(numBytes, largeMemorySize);
-	Block b;
-	b.base = _provider->allocate(size);
-	b.size = size;
-	// reuse a released slot so block indices of live records stay valid
-	for (size_t i = 0; i < _blocks.size(); i++) {
-		if (_blocks[i].base == 0) {
-			_blocks[i] = b;
-			return uint32_t(i);
-		}
-	}
-	_blocks.push_back(b);
-	return uint32_t(_blocks.size() - 1);
+		return at + numBytes <= b.size;
+	};
+	auto create = [&](size_t size) {
+		Block b;
+		b.base = _provider->allocate(size);
+		b.size = size;
+		_blocks.push_back(b);
+		return uint32_t(_blocks.size() - 1);
+	};
+	if (_firstBlock == ~0u)
+		_firstBlock = create(numBytes < smallMemorySize    ? smallMemorySize
+		                     : numBytes < mediumMemorySize ? mediumMemorySize
+		                                                   : std::max(numBytes, largeMemorySize));
+	if (fits(_firstBlock)) return _firstBlock;
+	if (_secondBlock == ~0u)
+		_secondBlock = create(numBytes < mediumMemorySize ? mediumMemorySize : std::max(numBytes, largeMemorySize));
+	if (fits(_secondBlock)) return _secondBlock;
+	const uint32_t b = create(std::max(numBytes, largeMemorySize));
+	_firstBlock = _secondBlock;
+	_secondBlock = b;
+	return b;
 }
 
 DataAllocationRecord* DataStorage::alloc(size_t numBytes)
@@ -174,9 +180,14 @@ DataAllocationRecord* DataStorage::realloc(DataAllocationRecord* record, size_t 
 void DataStorage::free(DataAllocationRecord* r) noexcept
 {
 	if (!r || r == &_zeroRecord || r->size == 0) return;
-	// drop staged bytes that were never uploaded
-	auto it = std::find(_pending.begin(), _pending.end(), r);
-	if (it != _pending.end()) _pending.erase(it);
+	// drop staged bytes that were never uploaded (the upload list's order
+	// does not matter: live allocations never overlap)
+	if (r->stagingData) {
+		DataAllocationRecord* last = _pending.back();
+		_pending[r->pendingIndex] = last;
+		last->pendingIndex = r->pendingIndex;
+		_pending.pop_back();
+	}
 	Block& b = _blocks[r->block];
 	if (--b.live == 0) b.top = 0;  // an empty block is reused from its start
 	_live--;
@@ -191,6 +202,7 @@ uint8_t* DataStorage::stage(DataAllocationRecord* r)
 		_stagingChunks.push_back(std::make_unique<uint8_t[]>(r->size));
 		r->stagingData = _stagingChunks.back().get();
 		std::memset(r->stagingData, 0, r->size);
+		r->pendingIndex = _pending.size();
 		_pending.push_back(r);
 	}
 	return r->stagingData;

@@ -49,6 +49,7 @@ struct DataAllocationRecord {
 	uint32_t block = ~0u;             // owning memory block
 	size_t stagingFrameNumber = size_t(-2);
 	uint8_t* stagingData = nullptr;   // host bytes to upload, valid until executeCopyOperations()
+	size_t pendingIndex = 0;          // position in the storage's upload list while stagingData is set
 };
 
 class CADR_EXPORT HandleTable {
@@ -132,6 +133,7 @@ private:
 		size_t live = 0;     // live allocations; the block is reset when it reaches 0
 	};
 	uint32_t blockFor(size_t numBytes, size_t align);
+	uint32_t _firstBlock = ~0u, _secondBlock = ~0u;  // allocation order of DataStorage.cpp:31-101
 
 	Renderer* _renderer;
 	MemoryProvider* _provider;

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -132,6 +133,91 @@ static void testAllocatorPlacement()
 			if (order == 0) { a1.free(); a2.free(); } else { a2.free(); a1.free(); }
 			r.executeCopyOperations();
 			CHECK(ds.liveAllocations() == 0, "storage empty after two frees");
+		}
+	}
+}
+
+// tests/DataAllocationTest.cpp:219-325: for every size s < 260 and every
+// count n with n allocations inside the first 64 KiB block, n allocations of
+// s bytes lie at the first address + i * offset(s) (the 16/64-byte ladder),
+// and every release order (in order, reverse, even then odd, odd then even,
+// both reversed, and after a 64 KiB - s allocation pushed the first one to
+// the end of the block) leaves the storage empty.
+static void testAllocatorSequences()
+{
+	Renderer r(false);
+	DataStorage& ds = r.dataStorage();
+	HandlelessAllocation probe(ds);
+	probe.alloc(10);
+	const uint64_t first = probe.deviceAddress();
+	probe.free();
+	r.executeCopyOperations();
+	auto empty = [&](const char* what, size_t s, size_t n) {
+		r.executeCopyOperations();
+		CHECK(ds.liveAllocations() == 0,
+		      std::string(what) + ": storage not empty (size " + std::to_string(s) + ", n " + std::to_string(n) + ")");
+	};
+	for (size_t s = 0; s < 260; s++) {
+		const size_t offset = (s == 0) ? 0 : (s <= 16) ? 16 : (s <= 32) ? 32 : (s <= 48) ? 48 : (s <= 64) ? 64
+		                                                                              : (s <= 128) ? 128
+		                                                                                           : (s + 63) & ~size_t(63);
+		for (size_t n = 0; n < 1030; n++) {
+			if (offset * n >= 65536) continue;
+			std::deque<HandlelessAllocation> v;
+			auto fill = [&](size_t count) {
+				for (size_t i = 0; i < count; i++) v.emplace_back(ds).alloc(s);
+			};
+			fill(n);
+			for (size_t i = 0; i < n; i++)
+				CHECK(v[i].deviceAddress() == first + offset * i || (s == 0 && v[i].deviceAddress() == 0),
+				      "allocation " + std::to_string(i) + " of " + std::to_string(n) + " of size " +
+				          std::to_string(s) + " at first + " + std::to_string(offset * i));
+			for (size_t i = 0; i < n; i++) v[i].free();
+			v.clear();
+			empty("in order", s, n);
+
+			fill(n);
+			for (size_t i = n; i > 0;) v[--i].free();
+			v.clear();
+			empty("reverse", s, n);
+
+			fill(n);
+			for (size_t i = 0; i < n; i += 2) v[i].free();
+			for (size_t i = 1; i < n; i += 2) v[i].free();
+			v.clear();
+			empty("even then odd", s, n);
+
+			fill(n);
+			for (size_t i = 1; i < n; i += 2) v[i].free();
+			for (size_t i = 0; i < n; i += 2) v[i].free();
+			v.clear();
+			empty("odd then even", s, n);
+
+			fill(n);
+			for (int64_t i = int64_t(n) - 2; i >= 0; i -= 2) v[size_t(i)].free();
+			for (int64_t i = int64_t(n) - 1; i >= 0; i -= 2) v[size_t(i)].free();
+			v.clear();
+			empty("every second reversed, from n-2", s, n);
+
+			fill(n);
+			for (int64_t i = int64_t(n) - 1; i >= 0; i -= 2) v[size_t(i)].free();
+			for (int64_t i = int64_t(n) - 2; i >= 0; i -= 2) v[size_t(i)].free();
+			v.clear();
+			empty("every second reversed, from n-1", s, n);
+
+			for (int reverse = 0; reverse < 2; reverse++) {
+				HandlelessAllocation big(ds);
+				big.alloc(65536 - s);
+				if (n != 0) fill(1);
+				big.free();
+				fill(n - (n != 0));
+				if (reverse)
+					for (size_t i = n; i > 0;) v[--i].free();
+				else
+					for (size_t i = 0; i < n; i++) v[i].free();
+				v.clear();
+				empty(reverse ? "after a 64 KiB allocation, reverse" : "after a 64 KiB allocation", s, n);
+			}
 		}
 	}
 }
@@ -459,6 +545,7 @@ int main(int argc, char** argv)
 		if (mode == "cpu") {
 			testHandleLevels();
 			testAllocatorPlacement();
+			testAllocatorSequences();
 			testStateSetOrder();
 		} else if (mode == "gpu") {
 			gpuFrame(1, 500, 1);
