@@ -84,7 +84,7 @@ int main(int argc, char** argv)
 		}
 		for (int k = 0; k <= K; k++) sc.stateSets.push_back(std::make_unique<StateSet>(r));
 		StateSet& root = *sc.stateSets[0];
-		for (int k = 1; k <= K; k++) root.appendChild(*sc.stateSets[k]);
+		for (int k = 1; k <= K; k++) root.childList.append(*sc.stateSets[k]);
 		sc.drawables.reserve(N);
 		for (size_t i = 0; i < N; i++)
 			sc.drawables.emplace_back(*sc.geometries[rng() % G], uint32_t(8 * (rng() % 4)), *sc.matrixLists[rng() % M],

@@ -1,5 +1,7 @@
 // Geometry, MatrixList, Drawable and StateSet of the CadR facade.
 #include <algorithm>
+#include <iterator>
+#include <list>
 #include <cstring>
 
 #include "../../../include/CadR/Drawable.h"
@@ -115,33 +117,58 @@ void Drawable::create(Geometry& g, uint32_t off, MatrixList& ml, DataAllocation&
 
 // -------------------------------------------------------------- StateSet
 
-StateSet::~StateSet() noexcept
+// Edges of the parent-child relation: one per link, in the parent's
+// childList and the child's parentList (ParentChildList.h).
+struct StateSetEdge {
+	StateSet* parent;
+	StateSet* child;
+	std::list<StateSetEdge*>::iterator inParent, inChild;
+};
+
+template <bool CHILDREN>
+StateSet& StateSetList<CHILDREN>::iterator::operator*() const
 {
-	removeAllDrawables();
-	for (StateSet* c : _children) {
-		auto& p = c->_parents;
-		p.erase(std::find(p.begin(), p.end(), this));
-	}
-	for (StateSet* p : _parents) {
-		auto& c = p->_children;
-		c.erase(std::remove(c.begin(), c.end(), this), c.end());
-	}
+	return CHILDREN ? *(*_it)->child : *(*_it)->parent;
 }
 
-void StateSet::appendChild(StateSet& child)
+template <bool CHILDREN>
+typename StateSetList<CHILDREN>::iterator StateSetList<CHILDREN>::append(StateSet& other)
 {
-	_children.push_back(&child);
-	child._parents.push_back(this);
+	auto* e = new StateSetEdge{CHILDREN ? _owner : &other, CHILDREN ? &other : _owner, {}, {}};
+	auto& children = e->parent->childList._list;
+	auto& parents = e->child->parentList._list;
+	try {
+		children.push_back(e);
+	} catch (...) {
+		delete e;
+		throw;
+	}
+	e->inParent = std::prev(children.end());
+	try {
+		parents.push_back(e);
+	} catch (...) {
+		children.erase(e->inParent);
+		delete e;
+		throw;
+	}
+	e->inChild = std::prev(parents.end());
+	return iterator(CHILDREN ? e->inParent : e->inChild);
 }
 
-void StateSet::removeChild(StateSet& child)
+template <bool CHILDREN>
+void StateSetList<CHILDREN>::remove(iterator it) noexcept
 {
-	auto it = std::find(_children.begin(), _children.end(), &child);
-	if (it == _children.end()) throw LogicError("CadR::StateSet::removeChild(): not a child");
-	_children.erase(it);
-	auto& p = child._parents;
-	p.erase(std::find(p.begin(), p.end(), this));
+	StateSetEdge* e = *it.raw();
+	e->parent->childList._list.erase(e->inParent);
+	e->child->parentList._list.erase(e->inChild);
+	delete e;
 }
+
+template class StateSetList<true>;
+template class StateSetList<false>;
+
+// The lists remove this StateSet's edges when they are destroyed.
+StateSet::~StateSet() noexcept { removeAllDrawables(); }
 
 void StateSet::markDirty(uint32_t index) noexcept
 {
@@ -211,9 +238,9 @@ size_t StateSet::prepareRecording()
 {
 	_skipRecording = !_forceRecording;
 	size_t n = _drawableDataList.size();
-	for (StateSet* c : _children) {
-		n += c->prepareRecording();
-		_skipRecording = _skipRecording && c->_skipRecording;
+	for (StateSet& c : childList) {
+		n += c.prepareRecording();
+		_skipRecording = _skipRecording && c._skipRecording;
 	}
 	_skipRecording = _skipRecording && n == 0;
 	return n;
@@ -230,7 +257,7 @@ void StateSet::recordToCommandBuffer(DrawableGpuData* staging, std::vector<DrawR
 		ranges.push_back(DrawRange{this, counter, n});
 		counter += n;
 	}
-	for (StateSet* c : _children) c->recordToCommandBuffer(staging, ranges, counter);
+	for (StateSet& c : childList) c.recordToCommandBuffer(staging, ranges, counter);
 }
 
 }  // namespace CadR

@@ -15,7 +15,10 @@
 
 #include "Export.h"
 
+#include <cstddef>
 #include <cstdint>
+#include <iterator>
+#include <list>
 #include <vector>
 
 #include "DataStorage.h"
@@ -112,6 +115,71 @@ private:
 	friend class StateSet;
 };
 
+// Parent-child relation of StateSets (ParentChildList.h): one edge per
+// parent -> child link, listed in the parent's childList and the child's
+// parentList in append order.  A StateSet may have several parents and may
+// be linked to the same child more than once.  childList.append(child) and
+// parentList.append(parent) create an edge and return the iterator to it in
+// that list; remove(iterator) deletes the edge from both lists; clear()
+// removes every edge of the list; destroying a StateSet removes its edges.
+// Dereferencing a childList iterator gives the child, a parentList iterator
+// the parent.
+struct StateSetEdge;
+
+template <bool CHILDREN>
+class CADR_EXPORT StateSetList {
+	using Raw = std::list<StateSetEdge*>;
+
+public:
+	class iterator {
+	public:
+		using iterator_category = std::bidirectional_iterator_tag;
+		using value_type = StateSet;
+		using difference_type = std::ptrdiff_t;
+		using pointer = StateSet*;
+		using reference = StateSet&;
+		iterator() = default;
+		explicit iterator(Raw::iterator it) : _it(it) {}
+		StateSet& operator*() const;
+		StateSet* operator->() const { return &**this; }
+		iterator& operator++() { ++_it; return *this; }
+		iterator operator++(int) { iterator t = *this; ++_it; return t; }
+		iterator& operator--() { --_it; return *this; }
+		iterator operator--(int) { iterator t = *this; --_it; return t; }
+		bool operator==(const iterator& o) const { return _it == o._it; }
+		bool operator!=(const iterator& o) const { return _it != o._it; }
+		Raw::iterator raw() const { return _it; }
+
+	private:
+		Raw::iterator _it;
+	};
+	using const_iterator = iterator;
+
+	explicit StateSetList(StateSet& owner) noexcept : _owner(&owner) {}
+	StateSetList(const StateSetList&) = delete;
+	StateSetList& operator=(const StateSetList&) = delete;
+	~StateSetList() { clear(); }
+
+	iterator append(StateSet& other);
+	void remove(iterator it) noexcept;
+	void clear() noexcept { while (!_list.empty()) remove(begin()); }
+	size_t size() const { return _list.size(); }
+	bool empty() const { return _list.empty(); }
+	StateSet& front() const { return *iterator(const_cast<Raw&>(_list).begin()); }
+	StateSet& back() const { return *std::prev(end()); }
+	iterator begin() const { return iterator(const_cast<Raw&>(_list).begin()); }
+	iterator end() const { return iterator(const_cast<Raw&>(_list).end()); }
+
+private:
+	StateSet* _owner;
+	Raw _list;
+	friend class StateSetList<!CHILDREN>;
+	friend class StateSet;
+};
+
+using ChildList = StateSetList<true>;
+using ParentList = StateSetList<false>;
+
 // One StateSet's share of the indirect buffers, as StateSet::recordToCommandBuffer
 // issues it: commands [base, base + count) (StateSet.cpp:233-265).
 struct DrawRange {
@@ -128,15 +196,15 @@ public:
 	StateSet& operator=(const StateSet&) = delete;
 
 	Renderer& renderer() const { return *_renderer; }
-	// parent-child relation (ParentChildList.h): a child may have several parents
-	void appendChild(StateSet& child);
-	void removeChild(StateSet& child);
-	const std::vector<StateSet*>& childList() const { return _children; }
-	const std::vector<StateSet*>& parentList() const { return _parents; }
+	// parent-child relation (StateSet.h:69-70, ParentChildList.h)
+	ChildList childList{*this};
+	ParentList parentList{*this};
 
 	// rendering (StateSet.cpp:180-268)
 	size_t prepareRecording();
 	void setForceRecording(bool v) { _forceRecording = v; }
+	bool forceRecording() const { return _forceRecording; }
+	void requestRecording() { _skipRecording = false; }
 	bool skipRecording() const { return _skipRecording; }
 	// appends this subtree's drawables (DFS) to the staging list and its draw ranges
 	void recordToCommandBuffer(DrawableGpuData* staging, std::vector<DrawRange>& ranges, size_t& drawableCounter);
@@ -159,8 +227,6 @@ private:
 	bool _forceRecording = false;
 	std::vector<DrawableGpuData> _drawableDataList;
 	std::vector<Drawable*> _drawablePtrList;
-	std::vector<StateSet*> _children;
-	std::vector<StateSet*> _parents;
 	// records changed since the renderer last uploaded this StateSet: every
 	// index whose record differs from the uploaded one at the same index is
 	// listed (append, removal's move into the hole, in-place create), or

@@ -2,7 +2,8 @@
 //
 //   facade_test cpu   device-less Renderer (host memory): handle numbering and
 //                     table levels, allocator placement (the checks of the
-//                     reference's tests/DataAllocationTest.cpp:55-130),
+//                     reference's tests/DataAllocationTest.cpp), the
+//                     parent / child lists (tests/ParentChildTest.cpp),
 //                     StateSet order, removal and DFS draw ranges
 //   facade_test gpu   Renderer on device 0: full frames through
 //                     prepareSceneRendering / recordDrawableProcessing at
@@ -230,6 +231,59 @@ struct Scene {
 	std::vector<std::unique_ptr<StateSet>> stateSets;
 };
 
+static void removeChildOnce(StateSet& parent, StateSet& child)
+{
+	for (auto it = parent.childList.begin(); it != parent.childList.end(); ++it)
+		if (&*it == &child) {
+			parent.childList.remove(it);
+			return;
+		}
+	throw std::runtime_error("removeChildOnce: not a child");
+}
+
+// tests/ParentChildTest.cpp: the childList / parentList operations, and the
+// relation seen from both ends.
+static void testParentChildLists()
+{
+	Renderer r(false);
+	StateSet ss1(r), ss2(r), ss3(r);
+	CHECK(ss1.childList.size() == 0 && ss1.childList.empty(), "empty child list");
+	ChildList::iterator it1 = ss1.childList.append(ss2);
+	CHECK(&*it1 == &ss2 && ss1.childList.size() == 1 && &ss1.childList.front() == &ss2 &&
+	          &ss1.childList.back() == &ss2,
+	      "child appended");
+	CHECK(ss2.parentList.size() == 1 && &ss2.parentList.front() == &ss1, "parent seen from the child");
+	ss1.childList.remove(it1);
+	CHECK(ss1.childList.empty() && ss2.parentList.empty(), "edge removed from both lists");
+	ss1.childList.append(ss2);
+	ss1.childList.append(ss3);
+	ss1.childList.append(ss2);  // a second link to the same child
+	size_t k = 0;
+	StateSet* expect[3] = {&ss2, &ss3, &ss2};
+	for (const StateSet& child : ss1.childList) {
+		CHECK(&child == expect[k] && &child.renderer() == &r, "child order is append order");
+		k++;
+	}
+	CHECK(k == 3 && &ss1.childList.back() == &ss2 && ss2.parentList.size() == 2, "three edges");
+	ss1.childList.clear();
+	CHECK(ss1.childList.empty() && ss2.parentList.empty() && ss3.parentList.empty(), "clear");
+
+	ParentList::iterator it2 = ss2.parentList.append(ss1);
+	CHECK(&*it2 == &ss1 && &ss1.childList.front() == &ss2 && &ss2.parentList.back() == &ss1, "parent appended");
+	ss2.parentList.append(ss3);
+	for (const StateSet& parent : ss2.parentList) parent.renderer();
+	ss2.parentList.remove(it2);
+	CHECK(ss2.parentList.size() == 1 && ss1.childList.empty() && ss3.childList.size() == 1, "parent edge removed");
+	ss2.parentList.clear();
+	CHECK(ss2.parentList.empty() && ss3.childList.empty(), "parent clear");
+	{
+		StateSet temp(r);
+		ss1.childList.append(temp);
+		temp.childList.append(ss3);
+	}
+	CHECK(ss1.childList.empty() && ss3.parentList.empty(), "a destroyed StateSet leaves no edges");
+}
+
 static void testStateSetOrder()
 {
 	Renderer r(false);
@@ -242,10 +296,10 @@ static void testStateSetOrder()
 	StateSet& a = *sc.stateSets[1];
 	StateSet& b = *sc.stateSets[2];
 	StateSet& shared = *sc.stateSets[3];
-	root.appendChild(a);
-	root.appendChild(b);
-	a.appendChild(shared);
-	b.appendChild(shared);  // two parents: visited once per path
+	root.childList.append(a);
+	root.childList.append(b);
+	a.childList.append(shared);
+	b.childList.append(shared);  // two parents: visited once per path
 	auto add = [&](StateSet& ss) {
 		sc.matrixLists.push_back(std::make_unique<MatrixList>(r));
 		float m[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
@@ -286,9 +340,9 @@ static void testStateSetOrder()
 	CHECK(std::memcmp(&staging[0], &a.drawableDataList()[0], 2 * sizeof(DrawableGpuData)) == 0, "staging content");
 	// an empty subtree is skipped; forcing recording keeps it
 	StateSet empty(r);
-	root.appendChild(empty);
+	root.childList.append(empty);
 	CHECK(root.prepareRecording() == 7 && empty.skipRecording(), "empty StateSet skipped");
-	root.removeChild(empty);
+	removeChildOnce(root, empty);
 	moved.destroy();
 }
 
@@ -320,7 +374,7 @@ static void gpuFrame(int handleLevel, size_t numDrawables, uint32_t seed)
 	}
 	const int K = 5;
 	for (int k = 0; k < K; k++) sc.stateSets.push_back(std::make_unique<StateSet>(r));
-	for (int k = 1; k < K; k++) sc.stateSets[k == 4 ? 2 : 0]->appendChild(*sc.stateSets[k]);
+	for (int k = 1; k < K; k++) sc.stateSets[k == 4 ? 2 : 0]->childList.append(*sc.stateSets[k]);
 	struct Expect {
 		int g, p;
 		size_t ml;
@@ -433,10 +487,10 @@ static void gpuResidentFrames(uint32_t seed)
 	const int K = 6;
 	for (int k = 0; k < K; k++) sc.stateSets.push_back(std::make_unique<StateSet>(r));
 	StateSet& root = *sc.stateSets[0];
-	root.appendChild(*sc.stateSets[1]);
-	root.appendChild(*sc.stateSets[2]);
-	sc.stateSets[1]->appendChild(*sc.stateSets[3]);
-	sc.stateSets[2]->appendChild(*sc.stateSets[3]);
+	root.childList.append(*sc.stateSets[1]);
+	root.childList.append(*sc.stateSets[2]);
+	sc.stateSets[1]->childList.append(*sc.stateSets[3]);
+	sc.stateSets[2]->childList.append(*sc.stateSets[3]);
 	struct Truth {
 		int g, p, ml;
 	};
@@ -460,11 +514,11 @@ static void gpuResidentFrames(uint32_t seed)
 		size_t touched = 0;  // records the edits of this frame changed (at most)
 		bool moved = false;  // the StateSet ranges may move
 		if (frame == 2) {
-			sc.stateSets[0]->appendChild(*sc.stateSets[4]);
+			sc.stateSets[0]->childList.append(*sc.stateSets[4]);
 			moved = true;
 		}
 		if (frame == 5) {
-			sc.stateSets[1]->removeChild(*sc.stateSets[3]);
+			removeChildOnce(*sc.stateSets[1], *sc.stateSets[3]);
 			moved = true;
 		}
 		if (frame == 7) r.setResidentDrawables(false);
@@ -546,6 +600,7 @@ int main(int argc, char** argv)
 			testHandleLevels();
 			testAllocatorPlacement();
 			testAllocatorSequences();
+			testParentChildLists();
 			testStateSetOrder();
 		} else if (mode == "gpu") {
 			gpuFrame(1, 500, 1);
