@@ -39,6 +39,8 @@ def lib() -> ctypes.CDLL:
             "oracle_lookup_handle": (_i, [_P, _u32, _u64, _u32, _u64, ctypes.POINTER(_u64)]),
             "oracle_mat4_mul": (None, [_P, _P, _P]),
             "oracle_sphere_transform": (None, [_P, _P, _P]),
+            "oracle_gltf_trs_matrix": (None, [_P, _P, _P, _P]),
+            "oracle_gltf_root_flip": (None, [_P, _P]),
             "oracle_propagate_level": (None, [_P, _P, _u32, _P, _P, _u64, _P, _P, _u32, _P]),
             "oracle_make_view": (_i, [_P, _P, _P, _f, _f, _f, _f, _f, _P, _P]),
             "oracle_test_drawable": (_i, [_P, _P, _P]),
@@ -66,6 +68,8 @@ def ref() -> ctypes.CDLL:
             "ref_look_at_lh": (None, [_P, _P, _P, _P]),
             "ref_perspective_lh_zo": (None, [_f, _f, _f, _f, _P]),
             "ref_propagate_tree": (None, [_P, _P, _u64, _P]),
+            "ref_gltf_trs_matrix": (None, [_P, _P, _P, _P]),
+            "ref_gltf_root_flip": (None, [_P, _P]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -94,6 +98,25 @@ def mat4_mul(A, B):
     R = np.zeros(16, dtype=np.float32)
     lib().oracle_mat4_mul(_p(np.ascontiguousarray(A, np.float32)), _p(np.ascontiguousarray(B, np.float32)), _p(R))
     return R
+
+
+def gltf_trs_matrix(scale, rotation=None, translation=None, ref=False):
+    """glTF node matrix (examples/gltfReader/main.cpp:1455-1500); ref=True runs
+    the reference's glm code from oracle/_ref instead of the restatement."""
+    s = np.ascontiguousarray(scale, np.float32)
+    q = None if rotation is None else np.ascontiguousarray(rotation, np.float32)
+    t = None if translation is None else np.ascontiguousarray(translation, np.float32)
+    M = np.zeros(16, dtype=np.float32)
+    f = globals()["ref"]().ref_gltf_trs_matrix if ref else lib().oracle_gltf_trs_matrix
+    f(_p(s), _p(q), _p(t), _p(M))
+    return M
+
+
+def gltf_root_flip(M, ref=False):
+    out = np.zeros(16, dtype=np.float32)
+    f = globals()["ref"]().ref_gltf_root_flip if ref else lib().oracle_gltf_root_flip
+    f(_p(np.ascontiguousarray(M, np.float32)), _p(out))
+    return out
 
 
 def sphere_transform(M, s):

@@ -139,6 +139,56 @@ static float std_max(float a, float b) { return (a < b) ? b : a; } /* std::max *
  * where the unqualified sqrt of BoundingSphere.h:84 is ::sqrt(double): the
  * root and the product are double, rounded to float by the float-returning
  * lambda. */
+/* glTF node matrix from scale s, rotation quaternion q {x,y,z,w} (NULL: none)
+ * and translation t (NULL: none), examples/gltfReader/main.cpp:1455-1500:
+ * with a rotation, the columns of glm::mat3(q) (mat3_cast,
+ * gtc/quaternion.inl:41-66) scaled by s.x, s.y, s.z, then w row 0,0,0,1; without
+ * one, diag(s.x, s.y, s.z, 1); the translation, when given, replaces column 3's
+ * x, y, z.  Column-major. */
+void oracle_gltf_trs_matrix(const float* s, const float* q, const float* t, float* M)
+{
+	memset(M, 0, 64);
+	if (q) {
+		const float qxx = q[0] * q[0], qyy = q[1] * q[1], qzz = q[2] * q[2];
+		const float qxz = q[0] * q[2], qxy = q[0] * q[1], qyz = q[1] * q[2];
+		const float qwx = q[3] * q[0], qwy = q[3] * q[1], qwz = q[3] * q[2];
+		float r[9];
+		r[0] = 1.f - 2.f * (qyy + qzz);
+		r[1] = 2.f * (qxy + qwz);
+		r[2] = 2.f * (qxz - qwy);
+		r[3] = 2.f * (qxy - qwz);
+		r[4] = 1.f - 2.f * (qxx + qzz);
+		r[5] = 2.f * (qyz + qwx);
+		r[6] = 2.f * (qxz + qwy);
+		r[7] = 2.f * (qyz - qwx);
+		r[8] = 1.f - 2.f * (qxx + qyy);
+		for (int c = 0; c < 3; c++)
+			for (int k = 0; k < 3; k++) M[4 * c + k] = r[3 * c + k] * s[c];
+	} else {
+		M[0] = s[0];
+		M[5] = s[1];
+		M[10] = s[2];
+	}
+	M[15] = 1.f;
+	if (t) {
+		M[12] = t[0];
+		M[13] = t[1];
+		M[14] = t[2];
+	}
+}
+
+/* Root matrix of a glTF scene, main.cpp:1547-1553: rows 1 and 2 (y and z)
+ * negated in every column. */
+void oracle_gltf_root_flip(const float* M, float* out)
+{
+	for (int c = 0; c < 4; c++) {
+		out[4 * c + 0] = M[4 * c + 0];
+		out[4 * c + 1] = -M[4 * c + 1];
+		out[4 * c + 2] = -M[4 * c + 2];
+		out[4 * c + 3] = M[4 * c + 3];
+	}
+}
+
 void oracle_sphere_transform(const float* M, const float* s, float* out)
 {
 	float c[3];

@@ -46,6 +46,10 @@ int oracle_lookup_handle(const oracle_region* regions, uint32_t numRegions, uint
 
 void oracle_mat4_mul(const float* A, const float* B, float* R);
 void oracle_sphere_transform(const float* M, const float* sphere, float* out);
+/* glTF node matrix from scale / rotation (NULL: none) / translation (NULL:
+ * none) and the root-node axis flip (examples/gltfReader/main.cpp:1455-1553). */
+void oracle_gltf_trs_matrix(const float* s, const float* q, const float* t, float* M);
+void oracle_gltf_root_flip(const float* M, float* out);
 /* One level: world = parentWorld[parent[i]] * local[i]; parentWorld == NULL
  * or parent[i] >= parentCount copies local.  spheres may be NULL. */
 void oracle_propagate_level(const float* local, const uint32_t* parent, uint32_t parentCount,

@@ -5,6 +5,8 @@
 #include <CadR/BoundingSphere.h>
 #include <glm/ext/matrix_clip_space.hpp>
 #include <glm/ext/matrix_transform.hpp>
+#include <glm/gtc/quaternion.hpp>
+#include <glm/mat3x3.hpp>
 #include <glm/mat4x4.hpp>
 
 #include <cstdint>
@@ -68,6 +70,49 @@ void ref_propagate_tree(const float* local, const int64_t* parent, uint64_t coun
 		                                  : loadMat(world + 16 * parent[i]) * loadMat(local + 16 * i);
 		storeMat(m, world + 16 * i);
 	}
+}
+
+// glTF node matrix as examples/gltfReader/main.cpp:1455-1500 builds it from
+// scale, rotation (glm::mat3 of a glm::quat) and translation; q or t may be
+// null (absent in the node).
+void ref_gltf_trs_matrix(const float* s, const float* q, const float* t, float* out)
+{
+	const glm::vec3 scale(s[0], s[1], s[2]);
+	glm::mat4 m(0.f);
+	if (q) {
+		glm::quat quat;
+		quat.x = q[0];
+		quat.y = q[1];
+		quat.z = q[2];
+		quat.w = q[3];
+		const glm::mat3 r = glm::mat3(quat);
+		m[0] = glm::vec4(r[0] * scale.x, 0.f);
+		m[1] = glm::vec4(r[1] * scale.y, 0.f);
+		m[2] = glm::vec4(r[2] * scale.z, 0.f);
+		m[3] = glm::vec4(0.f, 0.f, 0.f, 1.f);
+	} else {
+		m[0][0] = scale.x;
+		m[1][1] = scale.y;
+		m[2][2] = scale.z;
+		m[3][3] = 1.f;
+	}
+	if (t) {
+		m[3][0] = t[0];
+		m[3][1] = t[1];
+		m[3][2] = t[2];
+	}
+	storeMat(m, out);
+}
+
+// Root matrix of a glTF scene (main.cpp:1547-1553): y and z rows negated.
+void ref_gltf_root_flip(const float* M, float* out)
+{
+	glm::mat4 m = loadMat(M);
+	for (int i = 0; i < 4; i++) {
+		m[i][1] = -m[i][1];
+		m[i][2] = -m[i][2];
+	}
+	storeMat(m, out);
 }
 
 }  // extern "C"
