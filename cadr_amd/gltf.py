@@ -136,7 +136,10 @@ def load(doc) -> Hierarchy:
     scenes = doc.get("scenes", [])
     if not scenes:
         return h
-    scene = scenes[int(doc.get("scene", 0))]
+    si = doc.get("scene", 0)
+    if not isinstance(si, int) or not 0 <= si < len(scenes):
+        raise GltfError(f"scene index {si} out of range")
+    scene = scenes[si]
     local, parent, node_of, mesh_of = [], [], [], []
 
     def level(d):
@@ -145,10 +148,10 @@ def load(doc) -> Hierarchy:
 
     # iterative depth-first walk; a path longer than the node count has a cycle
     for root in scene.get("nodes", []):
-        stack = [(int(root), 0, -1)]
+        stack = [(root, 0, -1)]
         while stack:
             n, d, p = stack.pop()
-            if n < 0 or n >= len(nodes):
+            if not isinstance(n, int) or n < 0 or n >= len(nodes):
                 raise GltfError(f"node index {n} out of range")
             if d > len(nodes):
                 raise GltfError("the node hierarchy has a cycle")
@@ -158,12 +161,12 @@ def load(doc) -> Hierarchy:
             parent[d].append(p)
             node_of[d].append(n)
             m = nodes[n].get("mesh", -1)
-            if m != -1 and not 0 <= int(m) < len(meshes):
+            if m != -1 and not (isinstance(m, int) and 0 <= m < len(meshes)):
                 raise GltfError(f"mesh index {m} out of range")
             mesh_of[d].append(int(m))
             h.order.append((d, i))
             for c in reversed(nodes[n].get("children", [])):
-                stack.append((int(c), d + 1, i))
+                stack.append((c, d + 1, i))
     for d in range(len(local)):
         L = np.array(local[d], dtype=F).reshape(len(local[d]), 16)
         P = None if d == 0 else np.array(parent[d], dtype=np.uint32)

@@ -178,6 +178,10 @@ def test_malformed_nodes_raise():
         gltf.load(cyc)
     with pytest.raises(gltf.GltfError):
         gltf.load({"scenes": [{"nodes": [3]}], "nodes": [{}]})
+    with pytest.raises(gltf.GltfError):
+        gltf.load({"scene": 2, "scenes": [{"nodes": [0]}], "nodes": [{}]})
+    with pytest.raises(gltf.GltfError):
+        gltf.load({"scenes": [{"nodes": [0]}], "nodes": [{"mesh": 1}], "meshes": [{}]})
     assert gltf.load({"nodes": [{}]}).num_instances == 0  # no scene: nothing to draw
 
 
