@@ -40,6 +40,7 @@ def lib() -> ctypes.CDLL:
             "oracle_mat4_mul": (None, [_P, _P, _P]),
             "oracle_sphere_transform": (None, [_P, _P, _P]),
             "oracle_gltf_trs_matrix": (None, [_P, _P, _P, _P]),
+            "oracle_set_threads": (None, [ctypes.c_int]),
             "oracle_gltf_root_flip": (None, [_P, _P]),
             "oracle_propagate_level": (None, [_P, _P, _u32, _P, _P, _u64, _P, _P, _u32, _P]),
             "oracle_make_view": (_i, [_P, _P, _P, _f, _f, _f, _f, _f, _P, _P]),
@@ -124,6 +125,11 @@ def sphere_transform(M, s):
     lib().oracle_sphere_transform(_p(np.ascontiguousarray(M, np.float32)), _p(np.ascontiguousarray(s, np.float32)),
                                   _p(o))
     return o
+
+
+def set_threads(n: int) -> None:
+    """Threads of the propagation (the cull takes its own nthreads)."""
+    lib().oracle_set_threads(int(n))
 
 
 def propagate_level(local, parent, parent_world, mesh_index=None, meshes=None):

@@ -212,10 +212,17 @@ void oracle_sphere_transform(const float* M, const float* s, float* out)
 
 /* examples/gltfReader/main.cpp:1538-1576 (m = parentMatrix * node.matrix),
  * one tree level at a time. */
+/* Threads of oracle_propagate_level (1 by default); nodes are independent, so
+ * the results do not depend on it. */
+static int g_propagate_threads = 1;
+
+void oracle_set_threads(int n) { g_propagate_threads = n < 1 ? 1 : n; }
+
 void oracle_propagate_level(const float* local, const uint32_t* parent, uint32_t parentCount,
                             const float* parentWorld, float* world, uint64_t count, const uint32_t* meshIndex,
                             const cadrMesh* meshes, uint32_t numMeshes, float* spheres)
 {
+#pragma omp parallel for schedule(static) num_threads(g_propagate_threads) if (g_propagate_threads > 1)
 	for (uint64_t i = 0; i < count; i++) {
 		float* W = world + 16 * i;
 		if (parentWorld && parent[i] < parentCount)

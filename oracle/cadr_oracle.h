@@ -50,6 +50,8 @@ void oracle_sphere_transform(const float* M, const float* sphere, float* out);
  * none) and the root-node axis flip (examples/gltfReader/main.cpp:1455-1553). */
 void oracle_gltf_trs_matrix(const float* s, const float* q, const float* t, float* M);
 void oracle_gltf_root_flip(const float* M, float* out);
+/* Threads used by oracle_propagate_level (default 1). */
+void oracle_set_threads(int n);
 /* One level: world = parentWorld[parent[i]] * local[i]; parentWorld == NULL
  * or parent[i] >= parentCount copies local.  spheres may be NULL. */
 void oracle_propagate_level(const float* local, const uint32_t* parent, uint32_t parentCount,
