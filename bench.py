@@ -760,11 +760,17 @@ def cpu_baseline(scale: float = 0.25, budget_s: float = 12.0):
     finally:
         oracle.set_threads(1)
     t = float(np.median(times))
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), model)
+    except OSError:
+        pass
     log(f"cpu baseline: {t:.3f} s per {scene.num_drawables}-drawable frame ({len(times)} frames, {threads} threads)")
     return {"value": scene.num_drawables / t, "unit": "drawables/s", "cores": threads, "kind": "port",
             "sample": f"full10m structure at {scale:g} of the roots ({scene.num_drawables} drawables), whole frames "
                       f"(propagation + cull of 1 view), median of {len(times)}; oracle/cadr_oracle.c, gcc -O2, "
-                      f"{threads} OpenMP threads"}
+                      f"{threads} OpenMP threads of {os.cpu_count()} ({model})"}
 
 
 if __name__ == "__main__":
