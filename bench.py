@@ -154,8 +154,21 @@ def main():
     leaf_fused["spheres"] = 0  # the one-view frame needs no world spheres in HBM
     from cadr_amd.device import Event
 
+    # the levels above the leaf level; a root level's world matrices are its
+    # local ones, so the level below it reads the root's local matrices as its
+    # parents' world matrices and both go in one launch
+    upper = [lv.copy() for lv in level_descs[:-1]]
+    first_pair, first_pair_dev = None, None
+    if len(upper) >= 2 and int(upper[0]["parentWorld"][0]) == 0:
+        upper[1]["parentWorld"] = upper[0]["localMatrices"]
+        first_pair = np.concatenate(upper[:2])
+        first_pair_dev = device.DeviceBuffer.from_array(first_pair)
+        upper = upper[2:]
+
     def propagate_inner():
-        for lv in level_descs[:-1]:
+        if first_pair is not None:
+            N.check(lib.cadrPropagateTransformsBatch(first_pair_dev.address, N.ptr(first_pair), 2, s))
+        for lv in upper:
             N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
 
     # kernel shape of the fused leaf + cull launch: the fastest of the four
@@ -251,7 +264,8 @@ def main():
         "config": {"workload": "full10m: 1000x100x100 transform tree, 10M drawables, 64 StateSets, 3 LODs, "
                                "1 view (propagation + cull + LOD + per-StateSet compaction)",
                    "drawables": n, "nodes": int(sum(l[0] for l in ds.levels)), "statesets": ds.K, "views": 1,
-                   "frame": "2 propagation launches (root, inner level) + 1 fused leaf propagation + cull launch",
+                   "frame": "1 propagation launch (root and inner level together) + 1 fused leaf propagation + "
+                            "cull launch",
                    "block": fused_config.blockSize, "items_per_thread": fused_config.itemsPerThread,
                    "autotune": {"fused_ms": fused_table, "single_view": tuned1}},
         "roofline": {"bound": "hbm", "kernel": f"cullFusedKernel<{fused_config.blockSize},{fused_config.itemsPerThread}> "

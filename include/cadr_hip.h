@@ -235,9 +235,12 @@ int cadrPropagateTransforms(const cadrTransformLevel* level, cadrStream stream);
 /* Many independent levels (for example depth d of many trees) in one launch:
  * each computed exactly as cadrPropagateTransforms computes it.  deviceLevels
  * holds cadrTransformLevel[numLevels] in device memory, hostLevels the same
- * descriptors on the host (for checks).  No level may read another level of
- * the same launch; dependent levels go to successive launches on a stream.
- * numLevels <= 1024.  Capturable in a HIP graph. */
+ * descriptors on the host (for checks).  No level may read another level's
+ * world matrices in the same launch; dependent levels go to successive
+ * launches on a stream.  A root level's world matrices are its local ones,
+ * so a level may name a root level's localMatrices as its parentWorld and
+ * then share that root level's launch.  numLevels <= 1024.  Capturable in a
+ * HIP graph. */
 int cadrPropagateTransformsBatch(uint64_t deviceLevels, const cadrTransformLevel* hostLevels, uint32_t numLevels,
                                  cadrStream stream);
 

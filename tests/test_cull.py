@@ -402,3 +402,25 @@ def test_batched_propagation_matches_per_level(gpu):
         b.close()
     with pytest.raises(N.CadrError):
         N.check(N.load().cadrPropagateTransformsBatch(0, None, 1, None))
+
+
+def test_root_and_child_level_in_one_launch(gpu):
+    """A root level's world matrices equal its local ones, so the level below
+    can name the root's local matrices as parentWorld and share the root's
+    launch: every level bit for bit as the oracle's level-by-level result."""
+    from cadr_amd.device import DeviceBuffer, synchronize
+    sc = scenes.tree_scene((300, 40, 30), seed=scenes.SEED_BASE + 17, num_statesets=8)
+    ds = cull.DeviceScene(sc)
+    lv = ds.level_descs(spheres=True)
+    lv[1]["parentWorld"] = lv[0]["localMatrices"]
+    both = np.concatenate([lv[0], lv[1]])
+    dev = DeviceBuffer.from_array(both)
+    N.check(N.load().cadrPropagateTransformsBatch(dev.address, N.ptr(both), 2, None))
+    N.check(N.load().cadrPropagateTransforms(N.ptr(lv[2]), None))
+    synchronize()
+    worlds, sph = oracle.propagate_scene(sc)
+    for d in range(3):
+        assert ds.world_matrices(d).tobytes() == worlds[d].tobytes(), f"level {d}"
+    assert ds.download_spheres().tobytes() == sph.tobytes()
+    dev.close()
+    ds.close()
