@@ -287,7 +287,8 @@ def main():
 
     served_config = cull.make_config(args.block, args.items)
     if not args.no_batch:
-        result["batched"], served_config = bench_batched(args, ds, config, stream, lib, N, cull, scenes, device)
+        result["batched"], served_config = bench_batched(args, ds, config, stream, lib, N, cull, scenes, device,
+                                                         propagate_inner, level_descs[-1])
     if not args.no_uneven or not args.no_served or not args.no_gather:
         batch1.close(); batch_fused.close(); out1.close(); ds.close()
     if not args.no_gather:
@@ -330,9 +331,11 @@ def bench_resident(args, frames=10):
     return r
 
 
-def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
+def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device, propagate_inner, leaf_desc):
     """batch256: all views in one persistent launch (graph-replayed) vs the plain
-    pass (cadrCullView with the single-view config) launched once per view.
+    pass (cadrCullView with the single-view config) launched once per view;
+    then whole frames (propagation + all views): the leaf level as its own
+    launch before the batched cull, and fused into it (cadrPropagateCullBatch).
     Returns (result, batched config)."""
     from cadr_amd.tune import autotune, describe
     s = stream.handle
@@ -416,11 +419,54 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device):
     }
     log(f"batch{V}: {batch_ms:.3f} ms/frame ({tests / (batch_ms * 1e-3) / 1e9:.1f} G tests/s, {gbs:.0f} GB/s alg), "
         f"plain {plain_ms} ms, identical={same}")
+    res["frame"] = batched_frames(args, [(ds, views, out)], [leaf_desc], propagate_inner, batch, stream, lib, N, cull,
+                                  f"batch{V}")
     graph.close()
     batch.close(); out.close(); ws.close()
     for _, dj in plain_jobs:
         dj.close()
     return res, config
+
+
+def batched_frames(args, entries, leaf_descs, propagate_inner, batch, stream, lib, N, cull, name, unfused=None):
+    """Whole frames of a batched workload, each one HIP graph: the levels
+    above the leaves (propagate_inner), then either every job's leaf level as
+    its own launch (world spheres written) followed by the batched cull (or
+    the given unfused frame), or the leaf levels fused into the batched launch
+    (cadrPropagateCullBatch, no spheres in HBM), for each fused shape; counts
+    of both compared."""
+    s = stream.handle
+    steps = max(3, args.batch_steps)
+    leaves = np.concatenate(leaf_descs)
+
+    def separate():
+        propagate_inner()
+        for lv in leaf_descs:
+            N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
+        batch.run(s)
+
+    unfused = unfused or separate
+
+    un_ms = time_graph(unfused, stream, steps)
+    counts_un = [e[2].download()[1].copy() for e in entries]
+    fused_leaves = leaves.copy()
+    fused_leaves["spheres"] = 0
+    sweep, same = {}, True
+    for block, items in FUSED_BATCH_SHAPES:
+        b = cull.CullBatch(cull.make_config(block, items), entries)
+        b.set_leaves(fused_leaves)
+        for e in entries:
+            e[2].counts.upload(np.zeros(e[0].K * e[1].shape[0], np.uint32))
+        sweep[f"{block}x{items}"] = time_graph(lambda: (propagate_inner(), b.run_fused_batch(s)), stream, steps)
+        same = same and all(np.array_equal(e[2].download()[1], c) for e, c in zip(entries, counts_un))
+        b.close()
+    best = min(sweep, key=sweep.get)
+    r = {"unfused_ms_per_frame": un_ms, "fused_ms_per_frame": sweep[best], "fused_shape": best,
+         "fused_candidates_ms": sweep, "speedup_fused": un_ms / sweep[best], "fused_counts_identical": bool(same),
+         "frame": "levels above the leaves + (leaf propagation launch(es), batched cull) or the leaf levels fused "
+                  "into the batched launch; one HIP graph per frame"}
+    log(f"{name} frame: unfused {un_ms:.3f} ms, fused {sweep[best]:.3f} ms ({best}) {sweep}, identical={same}")
+    return r
 
 
 def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
@@ -465,6 +511,20 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
 
     def propagate_all():
         for dev, host in depth_batches:
+            N.check(lib.cadrPropagateTransformsBatch(dev.address, N.ptr(host), host.shape[0], s))
+
+    # the levels above each scene's leaf level, one launch per depth
+    inner_by_depth = {}
+    for lvs in levels:
+        for d, lv in enumerate(lvs[:-1]):
+            inner_by_depth.setdefault(d, []).append(lv)
+    inner_batches = []
+    for d in sorted(inner_by_depth):
+        host = np.concatenate(inner_by_depth[d])
+        inner_batches.append((device.DeviceBuffer.from_array(host), host))
+
+    def propagate_inner():
+        for dev, host in inner_batches:
             N.check(lib.cadrPropagateTransformsBatch(dev.address, N.ptr(host), host.shape[0], s))
 
     # plain: one single-view launch per (scene, view), same outputs, tiles
@@ -550,13 +610,16 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     }
     log(f"uneven64: frame {frame_ms:.3f} ms, cull {cull_ms:.3f} ms ({cull_gbs:.0f} GB/s alg), "
         f"plain cull {plain_ms} ms, identical={same}")
+    res["fused_frame"] = batched_frames(args, entries, [lvs[-1] for lvs in levels], propagate_inner, batch, stream,
+                                        lib, N, cull, "uneven64", unfused=lambda: (propagate_all(), batch.run(s)))
+    res["fused_frame"]["culled_drawables_per_s"] = tests / (res["fused_frame"]["fused_ms_per_frame"] * 1e-3)
     frame.close(); cull_only.close()
     for _, dj in plain:
         dj.close()
     batch.close(); ws.close()
     for ds, _, out in entries:
         out.close(); ds.close()
-    for dev, _ in depth_batches:
+    for dev, _ in depth_batches + inner_batches:
         dev.close()
     return res
 
@@ -711,6 +774,33 @@ def time_plain(run_plain, stream, reps):
         stream.synchronize()
         times.append(e0.elapsed_ms(e1))
     return min(times), sum(times) / len(times)
+
+
+def time_graph(fn, stream, steps):
+    """fn's launches captured in one HIP graph; 2 warm-up replays, then
+    `steps` replays timed with HIP events on the stream; ms per replay."""
+    from cadr_amd.device import Event
+    s = stream.handle
+    g = Graph()
+    g.begin(s)
+    fn()
+    g.end(s)
+    try:
+        for _ in range(2):
+            g.launch(s)
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(s)
+        for _ in range(steps):
+            g.launch(s)
+        e1.record(s)
+        stream.synchronize()
+        return e0.elapsed_ms(e1) / steps
+    finally:
+        g.close()
+
+
+FUSED_BATCH_SHAPES = ((256, 4), (256, 8), (512, 4))
 
 
 def tune_key(t):

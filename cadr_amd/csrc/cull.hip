@@ -76,7 +76,8 @@ struct CullParams {
 	uint32_t numJobs;
 	uint32_t totalWork;
 	uint32_t flags;
-	LeafParams leaf;
+	LeafParams leaf;                    // one-view fused kernel
+	const cadrTransformLevel* leaves;   // fused batched kernel: leaf level of each job
 };
 
 // Addresses read from descriptors would be generic (FLAT) pointers, and a
@@ -157,6 +158,26 @@ __device__ __forceinline__ cadrCullJob loadJob(const cadrCullJob* jobs, uint32_t
 	J.numDrawables = w[0]; J.numTiles = w[1]; J.numStateSets = w[2];
 	J.numViews = w[3]; J.commandStride = w[4]; J.numMeshes = w[5];
 	return J;
+}
+
+// Leaf level of job j for the fused batched kernel (cadrTransformLevel in
+// device memory, scalar loads); the parent count applies to a non-root level
+// only, as in the one-view fused entry point.
+static_assert(offsetof(cadrTransformLevel, numMeshes) == 64 && offsetof(cadrTransformLevel, parentCount) == 68 &&
+                  sizeof(cadrTransformLevel) == 72,
+              "cadrTransformLevel layout");
+__device__ __forceinline__ LeafParams loadLeaf(const cadrTransformLevel* leaves, uint32_t j)
+{
+	const uint64_t a = addr(leaves) + uint64_t(j) * sizeof(cadrTransformLevel);
+	const cptr<uint64_t> q = constPtr<uint64_t>(a);
+	LeafParams L;
+	L.local = q[0];
+	L.parentIndex = q[1];
+	L.parentWorld = q[2];
+	L.world = q[3];
+	L.spheres = q[7];
+	L.parentCount = L.parentWorld ? constPtr<uint32_t>(a + 68)[0] : 0u;
+	return L;
 }
 
 __device__ __forceinline__ TileDesc loadTile(const cadrCullJob& J, uint32_t t, uint32_t tileSize)
@@ -480,9 +501,11 @@ __device__ __forceinline__ bool stageMeshes(uint32_t* sMeshes, const cadrCullJob
 // matrices of item j are one contiguous 4-KiB block, loaded lane-contiguous
 // (each instruction one KiB) and redistributed through LDS.  Parent and mesh
 // indices of all items are loaded first (clamped into the tile, like
-// loadTileItems); the local block and parent matrix of item j+1 are issued
-// before item j is computed, so each item's loads overlap the previous
-// item's work.
+// loadTileItems); with PREFETCH the local block and parent matrix of item
+// j+1 are issued before item j is computed, so each item's loads overlap the
+// previous item's work.  The batched kernel prefetches only with 8 items per
+// thread: with 4, the 32 registers of the prefetch drop it from 3 to 2 waves
+// per SIMD for the whole view loop (measured slower).
 struct ItemLoads {
 	float4 raw[4];  // this lane's 4 float4 of the wave's 4-KiB local block
 	float4 P[4];    // this lane's parent world matrix
@@ -507,7 +530,7 @@ __device__ __forceinline__ void issueItemLoads(const LeafParams& Lf, uint64_t fi
 	}
 }
 
-template <int ITEMS, bool LDS>
+template <int ITEMS, bool LDS, bool PREFETCH = true>
 __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const cadrCullJob& J, const TileDesc& T,
                                                    uint32_t base, uint32_t lane, float4* stage,
                                                    const MeshTable& meshes, float4 (&sp)[ITEMS],
@@ -538,8 +561,8 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 	// items whose 64 nodes all lie past the tile end come last and are empty
 	const uint32_t items = base >= T.count ? 0u : min(uint32_t(ITEMS), (T.count - base + 63u) / 64u);
 	ItemLoads next;
-	if (items > 0) issueItemLoads(Lf, uint64_t(T.start) + base, end, pi[0], uint64_t(T.start) + base + lane < end,
-	                              lane, next);
+	if (PREFETCH && items > 0)
+		issueItemLoads(Lf, uint64_t(T.start) + base, end, pi[0], uint64_t(T.start) + base + lane < end, lane, next);
 #pragma unroll
 	for (int j = 0; j < ITEMS; j++) {
 		if (uint32_t(j) >= items) {
@@ -549,9 +572,15 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 		const uint64_t first = uint64_t(T.start) + base + uint32_t(j) * 64u;
 		const uint64_t i = first + lane;
 		const bool live = i < end;
-		const ItemLoads cur = next;
-		if (uint32_t(j) + 1u < items)
-			issueItemLoads(Lf, first + 64u, end, pi[j + 1 < ITEMS ? j + 1 : j], first + 64u + lane < end, lane, next);
+		ItemLoads cur;
+		if constexpr (PREFETCH) {
+			cur = next;
+			if (uint32_t(j) + 1u < items)
+				issueItemLoads(Lf, first + 64u, end, pi[j + 1 < ITEMS ? j + 1 : j], first + 64u + lane < end, lane,
+				               next);
+		} else {
+			issueItemLoads(Lf, first, end, pi[j], live, lane, cur);
+		}
 		// redistribute the local block: lane l takes matrix l
 #pragma unroll
 		for (int k = 0; k < 4; k++) stage[stageSlot(uint32_t(k) * 64u + lane)] = cur.raw[k];
@@ -575,6 +604,9 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 		if (live && mi[j] < J.numMeshes) s = sphereTransform(W, meshSphere<LDS>(meshes, mi[j]));
 		if (Lf.spheres && live) storeF4(globalMut<float4>(Lf.spheres) + i, s);
 		sp[j] = s;
+		// without the prefetch, keep the scheduler from hoisting later items'
+		// loads above this one (every item's 32 registers of loads at once)
+		if constexpr (!PREFETCH) __builtin_amdgcn_sched_barrier(0);
 	}
 }
 
@@ -731,13 +763,14 @@ struct CullShared {
 	uint32_t ticket;
 };
 
-template <int BLOCK, int ITEMS>
+template <int BLOCK, int ITEMS, bool FUSED>
 __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 {
 	using S = CullShared<BLOCK, ITEMS>;
 	constexpr int WAVES = S::WAVES;
 	constexpr int WAVE_ITEMS = 64 * ITEMS;
 	__shared__ S sh;
+	__shared__ float4 sMat[FUSED ? WAVES : 1][FUSED ? 256 : 1];  // local-matrix staging of the fused form
 
 	const uint32_t tid = threadIdx.x;
 	const uint32_t lane = laneId();
@@ -765,12 +798,26 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		const TileDesc T = loadTile(J, t, S::TILE);
 		const bool ldsMeshes = stageMeshes<BLOCK>(sh.meshes, J, tid);
 
-		// load the tile into registers (wave-contiguous runs) and bound it for the tile sphere
-		const gptr<float4> spheres = globalPtr<float4>(J.spheres);
-		const gptr<uint32_t> meshIndex = globalPtr<uint32_t>(J.meshIndex);
+		const MeshTable meshes{sh.meshes, globalPtr<cadrMesh>(J.meshes), ldsMeshes};
+
+		// load the tile into registers (wave-contiguous runs) and bound it for
+		// the tile sphere; the fused form computes the spheres from the job's
+		// leaf level instead, writing its world matrices on the way
 		float4 sp[ITEMS];
 		uint32_t mi[ITEMS];
-		loadTileItems<ITEMS>(spheres, meshIndex, T, wave * WAVE_ITEMS + lane, sp, mi);
+		if constexpr (FUSED) {
+			__syncthreads();  // the sphere transforms read the LDS mesh table
+			const LeafParams Lf = loadLeaf(P.leaves, job);
+			constexpr bool PREFETCH = ITEMS >= 8;
+			float4* stage = &sMat[wave][0];
+			if (ldsMeshes)
+				propagateTileItems<ITEMS, true, PREFETCH>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp, mi);
+			else
+				propagateTileItems<ITEMS, false, PREFETCH>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp, mi);
+		} else {
+			loadTileItems<ITEMS>(globalPtr<float4>(J.spheres), globalPtr<uint32_t>(J.meshIndex), T,
+			                     wave * WAVE_ITEMS + lane, sp, mi);
+		}
 		bool irregular = false;
 		float mn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
 		float mx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
@@ -863,7 +910,6 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 
 		const uint64_t views = J.views;
 		uint64_t* chain0 = P.status + constPtr<uint64_t>(addr(P.statusBase))[job];
-		const MeshTable meshes{sh.meshes, globalPtr<cadrMesh>(J.meshes), ldsMeshes};
 
 		for (uint32_t g0 = 0; g0 < J.numViews; g0 += kViewGroup) {
 			const uint32_t vg = uniform(min(kViewGroup, J.numViews - g0));
@@ -1100,22 +1146,23 @@ using KernelFn = void (*)(CullParams);
 
 struct KernelEntry {
 	uint32_t block, items;
-	KernelFn batch;   // many views / jobs
-	KernelFn view;    // one view (256-thread blocks only), or nullptr
-	KernelFn fused;   // leaf propagation + one view, or nullptr
+	KernelFn batch;       // many views / jobs
+	KernelFn view;        // one view (256-thread blocks only), or nullptr
+	KernelFn fused;       // leaf propagation + one view, or nullptr
+	KernelFn fusedBatch;  // leaf propagation + many views / jobs, or nullptr
 };
 
 const KernelEntry kKernels[] = {
-    {256, 2, cullKernel<256, 2>, cullViewKernel<2>, cullFusedKernel<256, 2, 4>},
-    {256, 4, cullKernel<256, 4>, cullViewKernel<4>, cullFusedKernel<256, 4, 4>},
-    {256, 8, cullKernel<256, 8>, cullViewKernel<8>, cullFusedKernel<256, 8, 1>},
-    {256, 16, cullKernel<256, 16>, cullViewKernel<16>, cullFusedKernel<256, 16, 1>},
-    {512, 2, cullKernel<512, 2>, nullptr, cullFusedKernel<512, 2, 1>},
-    {512, 4, cullKernel<512, 4>, nullptr, cullFusedKernel<512, 4, 1>},
-    {1024, 1, nullptr, nullptr, cullFusedKernel<1024, 1, 1>},
-    {1024, 2, nullptr, nullptr, cullFusedKernel<1024, 2, 1>},
-    {128, 8, cullKernel<128, 8>, nullptr, nullptr},
-    {128, 16, cullKernel<128, 16>, nullptr, nullptr},
+    {256, 2, cullKernel<256, 2, false>, cullViewKernel<2>, cullFusedKernel<256, 2, 4>, nullptr},
+    {256, 4, cullKernel<256, 4, false>, cullViewKernel<4>, cullFusedKernel<256, 4, 4>, cullKernel<256, 4, true>},
+    {256, 8, cullKernel<256, 8, false>, cullViewKernel<8>, cullFusedKernel<256, 8, 1>, cullKernel<256, 8, true>},
+    {256, 16, cullKernel<256, 16, false>, cullViewKernel<16>, cullFusedKernel<256, 16, 1>, nullptr},
+    {512, 2, cullKernel<512, 2, false>, nullptr, cullFusedKernel<512, 2, 1>, nullptr},
+    {512, 4, cullKernel<512, 4, false>, nullptr, cullFusedKernel<512, 4, 1>, cullKernel<512, 4, true>},
+    {1024, 1, nullptr, nullptr, cullFusedKernel<1024, 1, 1>, nullptr},
+    {1024, 2, nullptr, nullptr, cullFusedKernel<1024, 2, 1>, nullptr},
+    {128, 8, cullKernel<128, 8, false>, nullptr, nullptr, nullptr},
+    {128, 16, cullKernel<128, 16, false>, nullptr, nullptr, nullptr},
 };
 
 const KernelEntry* findKernel(const cadrCullConfig* c)
@@ -1229,16 +1276,32 @@ CADR_API int cadrCullWorkspaceDestroy(cadrCullWorkspace* w)
 	return ok();
 }
 
-CADR_API int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, const cadrCullJob* hostJobs,
-                           uint32_t numJobs, cadrCullWorkspace* ws, cadrStream stream)
+namespace {
+
+// Leaf level of a fused launch against its job: node i is drawable i.
+int checkLeaf(const cadrTransformLevel& L, const cadrCullJob& J, const std::string& fn)
 {
-	const KernelEntry* ke = findKernel(config);
-	CADR_REQUIRE(ke && ke->batch, "cadrCullBatch: unsupported blockSize/itemsPerThread");
-	CADR_REQUIRE(ws, "cadrCullBatch: workspace is NULL");
+	CADR_REQUIRE(L.count == J.numDrawables, fn + ": the leaf level must hold the job's drawables");
+	CADR_REQUIRE(L.count == 0 || (L.localMatrices && L.worldMatrices), fn + ": null matrix array");
+	CADR_REQUIRE((L.localMatrices % 16) == 0 && (L.worldMatrices % 16) == 0 && (L.parentWorld % 16) == 0 &&
+	                 (L.spheres % 16) == 0,
+	             fn + ": matrix and sphere arrays must be 16-byte aligned");
+	CADR_REQUIRE(L.parentWorld == 0 || (L.parentIndex && L.parentIndex % 4 == 0),
+	             fn + ": parentIndex is NULL or misaligned for a non-root level");
+	return CADR_SUCCESS;
+}
+
+// One persistent launch of a batched kernel over numJobs jobs; leaves: the
+// fused form's leaf levels (device memory), else null.
+int launchBatch(KernelFn fn, const KernelEntry& ke, const cadrCullConfig* config, uint64_t deviceJobs,
+                const cadrCullJob* hostJobs, uint32_t numJobs, cadrCullWorkspace* ws, cadrStream stream,
+                const cadrTransformLevel* leaves, const std::string& name)
+{
+	CADR_REQUIRE(ws, name + ": workspace is NULL");
 	if (numJobs == 0) return ok();
-	CADR_REQUIRE(deviceJobs && hostJobs, "cadrCullBatch: null job array");
-	CADR_REQUIRE(deviceJobs % 16 == 0, "cadrCullBatch: deviceJobs must be 16-byte aligned");
-	CADR_REQUIRE(numJobs <= ws->maxJobs, "cadrCullBatch: more jobs than the workspace was created for");
+	CADR_REQUIRE(deviceJobs && hostJobs, name + ": null job array");
+	CADR_REQUIRE(deviceJobs % 16 == 0, name + ": deviceJobs must be 16-byte aligned");
+	CADR_REQUIRE(numJobs <= ws->maxJobs, name + ": more jobs than the workspace was created for");
 	uint64_t totalWork = 0, statusNeed = 0;
 	for (uint32_t j = 0; j < numJobs; j++) {
 		const int rc = checkJob(hostJobs[j]);
@@ -1246,11 +1309,11 @@ CADR_API int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, co
 		totalWork += hostJobs[j].numTiles;
 		statusNeed += uint64_t(hostJobs[j].numViews) * hostJobs[j].numTiles;
 	}
-	CADR_REQUIRE(totalWork < 0xffffffffull, "cadrCullBatch: too many tiles in one launch");
-	CADR_REQUIRE(statusNeed <= ws->statusWords, "cadrCullBatch: workspace has too few status words");
+	CADR_REQUIRE(totalWork < 0xffffffffull, name + ": too many tiles in one launch");
+	CADR_REQUIRE(statusNeed <= ws->statusWords, name + ": workspace has too few status words");
 	int dev = 0;
 	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
-	CADR_REQUIRE(dev == ws->device, "cadrCullBatch: workspace belongs to another device");
+	CADR_REQUIRE(dev == ws->device, name + ": workspace belongs to another device");
 
 	hipStream_t s = hipStream_t(stream);
 	const auto* jobs = reinterpret_cast<const cadrCullJob*>(deviceJobs);
@@ -1261,10 +1324,10 @@ CADR_API int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, co
 
 	const int cu = computeUnitCount();
 	uint32_t perCU = config->workgroupsPerCU;
-	if (perCU == 0) perCU = ke->block >= 512 ? 2 : 4;
+	if (perCU == 0) perCU = ke.block >= 512 ? 2 : 4;
 	uint64_t grid = uint64_t(cu > 0 ? cu : 256) * perCU;
 	if (grid > totalWork) grid = totalWork;
-	CullParams p;
+	CullParams p{};
 	p.jobs = jobs;
 	p.firstWork = ws->firstWork;
 	p.statusBase = ws->statusBase;
@@ -1273,9 +1336,38 @@ CADR_API int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, co
 	p.numJobs = numJobs;
 	p.totalWork = uint32_t(totalWork);
 	p.flags = config->flags;
-	hipLaunchKernelGGL(ke->batch, dim3(uint32_t(grid)), dim3(ke->block), 0, s, p);
+	p.leaves = leaves;
+	hipLaunchKernelGGL(fn, dim3(uint32_t(grid)), dim3(ke.block), 0, s, p);
 	CADR_CHECK_HIP(hipGetLastError(), "cullKernel launch");
 	return ok();
+}
+
+}  // namespace
+
+CADR_API int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, const cadrCullJob* hostJobs,
+                           uint32_t numJobs, cadrCullWorkspace* ws, cadrStream stream)
+{
+	const KernelEntry* ke = findKernel(config);
+	CADR_REQUIRE(ke && ke->batch, "cadrCullBatch: unsupported blockSize/itemsPerThread");
+	return launchBatch(ke->batch, *ke, config, deviceJobs, hostJobs, numJobs, ws, stream, nullptr, "cadrCullBatch");
+}
+
+CADR_API int cadrPropagateCullBatch(uint64_t deviceLeaves, const cadrTransformLevel* hostLeaves,
+                                    const cadrCullConfig* config, uint64_t deviceJobs, const cadrCullJob* hostJobs,
+                                    uint32_t numJobs, cadrCullWorkspace* ws, cadrStream stream)
+{
+	const KernelEntry* ke = findKernel(config);
+	CADR_REQUIRE(ke && ke->fusedBatch,
+	             "cadrPropagateCullBatch: unsupported blockSize/itemsPerThread (256 x 4/8, 512 x 4)");
+	if (numJobs == 0) return ok();
+	CADR_REQUIRE(deviceLeaves && hostLeaves && hostJobs, "cadrPropagateCullBatch: null leaf or job array");
+	CADR_REQUIRE(deviceLeaves % 8 == 0, "cadrPropagateCullBatch: deviceLeaves must be 8-byte aligned");
+	for (uint32_t j = 0; j < numJobs; j++) {
+		const int rc = checkLeaf(hostLeaves[j], hostJobs[j], "cadrPropagateCullBatch");
+		if (rc != CADR_SUCCESS) return rc;
+	}
+	return launchBatch(ke->fusedBatch, *ke, config, deviceJobs, hostJobs, numJobs, ws, stream,
+	                   reinterpret_cast<const cadrTransformLevel*>(deviceLeaves), "cadrPropagateCullBatch");
 }
 
 CADR_API int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCullJob* hostJob,
@@ -1316,14 +1408,9 @@ CADR_API int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCul
 	             "cadrPropagateCullView: unsupported blockSize/itemsPerThread (256 x 2/4/8/16, 512 x 2/4, 1024 x 1/2)");
 	CADR_REQUIRE(ws, "cadrPropagateCullView: workspace is NULL");
 	CADR_REQUIRE(deviceJob && deviceJob % 16 == 0, "cadrPropagateCullView: deviceJob must be a 16-byte aligned address");
-	CADR_REQUIRE(leaf->count == hostJob->numDrawables, "cadrPropagateCullView: the leaf level must hold the job's drawables");
-	CADR_REQUIRE(leaf->localMatrices && leaf->worldMatrices, "cadrPropagateCullView: null matrix array");
-	CADR_REQUIRE((leaf->localMatrices % 16) == 0 && (leaf->worldMatrices % 16) == 0 && (leaf->parentWorld % 16) == 0 &&
-	                 (leaf->spheres % 16) == 0,
-	             "cadrPropagateCullView: matrix and sphere arrays must be 16-byte aligned");
-	CADR_REQUIRE(leaf->parentWorld == 0 || (leaf->parentIndex && leaf->parentIndex % 4 == 0),
-	             "cadrPropagateCullView: parentIndex is NULL or misaligned for a non-root level");
-	const int rc = checkJob(*hostJob);
+	int rc = checkLeaf(*leaf, *hostJob, "cadrPropagateCullView");
+	if (rc != CADR_SUCCESS) return rc;
+	rc = checkJob(*hostJob);
 	if (rc != CADR_SUCCESS) return rc;
 	CADR_REQUIRE(hostJob->numTiles <= ws->statusWords, "cadrPropagateCullView: workspace has too few status words");
 	int dev = 0;

@@ -261,8 +261,29 @@ class CullBatch:
         N.check(N.load().cadrPropagateCullView(N.ptr(leaf), ctypes.byref(self.config), self.device_jobs.address,
                                                N.ptr(self.host_jobs), self.workspace.handle, stream))
 
+    def set_leaves(self, leaves: np.ndarray) -> None:
+        """Leaf levels (LEVEL_DTYPE, one per job) for run_fused_batch; kept
+        in device memory until close()."""
+        leaves = np.ascontiguousarray(leaves, dtype=N.LEVEL_DTYPE)
+        assert leaves.shape == self.host_jobs.shape
+        if getattr(self, "_leaves_dev", None) is not None:
+            self._leaves_dev.close()
+        self.host_leaves = leaves
+        self._leaves_dev = DeviceBuffer.from_array(leaves)
+
+    def run_fused_batch(self, stream=None) -> None:
+        """cadrPropagateCullBatch: each job's leaf level (set_leaves) fused
+        with the batched cull of all jobs, one persistent launch."""
+        N.check(N.load().cadrPropagateCullBatch(self._leaves_dev.address, N.ptr(self.host_leaves),
+                                                ctypes.byref(self.config), self.device_jobs.address,
+                                                N.ptr(self.host_jobs), self.host_jobs.shape[0],
+                                                self.workspace.handle, stream))
+
     def close(self):
         self.device_jobs.close()
+        if getattr(self, "_leaves_dev", None) is not None:
+            self._leaves_dev.close()
+            self._leaves_dev = None
         for b in self._keep:
             b.close()
         if self._own_ws:

@@ -380,6 +380,23 @@ int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCul
 int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCullConfig* config, uint64_t deviceJob,
                           const cadrCullJob* hostJob, cadrCullWorkspace* workspace, cadrStream stream);
 
+/* The leaf level of every job fused with the batched cull, in one persistent
+ * launch: cadrPropagateTransformsBatch(leaves) followed by cadrCullBatch(jobs),
+ * tile by tile, each tile's world matrices computed once for all of its job's
+ * views (the propagation of examples/gltfReader/main.cpp:1538-1576 feeding the
+ * per-frame pass that Renderer::recordDrawableProcessing, Renderer.cpp:598-720,
+ * records).  deviceLeaves holds cadrTransformLevel[numJobs] in device memory,
+ * hostLeaves the same descriptors on the host (for checks); leaf j describes
+ * job j exactly as the leaf of cadrPropagateCullView does (leaf count ==
+ * numDrawables, spheres from the job's meshIndex and meshes, written only when
+ * the leaf's spheres != 0).  World matrices are written for the drawables the
+ * job's tiles cover (all of them for tiles from cadrBuildCullTiles at this
+ * tile size).  Results are bit-identical to the two separate calls.  config:
+ * 256 x 4/8 or 512 x 4.  Capturable in a HIP graph. */
+int cadrPropagateCullBatch(uint64_t deviceLeaves, const cadrTransformLevel* hostLeaves,
+                           const cadrCullConfig* config, uint64_t deviceJobs, const cadrCullJob* hostJobs,
+                           uint32_t numJobs, cadrCullWorkspace* workspace, cadrStream stream);
+
 /* Kernel tile size for a config (blockSize * itemsPerThread), 0 if unsupported. */
 uint32_t cadrCullTileSize(const cadrCullConfig* config);
 

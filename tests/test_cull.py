@@ -424,3 +424,95 @@ def test_root_and_child_level_in_one_launch(gpu):
     assert ds.download_spheres().tobytes() == sph.tobytes()
     dev.close()
     ds.close()
+
+
+def _fused_batch_cases():
+    """Scenes for the fused batched launch: tree depths 1 (flat: the leaf is
+    a root level) to 4, LOD counts 1 to 4, 1 to 65 views (two view groups),
+    a mesh table too large for LDS, and a scene with invalid mesh indices and
+    parent indices past the parent level (those nodes act as roots)."""
+    out = []
+    for i, (n, depth, lods, nv) in enumerate([(3000, 1, 1, 3), (50_000, 2, 4, 17), (90_000, 3, 2, 1), (1, 1, 1, 5),
+                                              (40_000, 4, 3, 65)]):
+        out.append(scenes.uneven_scene(60 + i, n, depth, lods, nv))
+    sc = scenes.tree_scene((300, 130), seed=scenes.SEED_BASE + 33, num_statesets=5, lod_count=2, num_meshes=100,
+                           trange=200.0)
+    sc.cameras = scenes.random_cameras(scenes.SEED_BASE + 35, 20, 0, 9, box=300.0)
+    out.append(sc)
+    sc = scenes.uneven_scene(70, 20_000, 2, 3, 7)
+    rng = np.random.default_rng(8)
+    mi = sc.mesh_index.copy()
+    mi[rng.integers(0, mi.shape[0], 500)] = 64 + rng.integers(0, 1000, 500).astype(np.uint32)
+    local, parent = sc.levels[1]
+    parent = parent.copy()
+    parent[rng.integers(0, parent.shape[0], 700)] = sc.levels[0][0].shape[0] + rng.integers(0, 5, 700).astype(np.uint32)
+    sc.mesh_index = mi
+    sc.levels[1] = (local, parent)
+    out.append(sc)
+    return out
+
+
+def test_fused_batch_propagation_and_cull(gpu):
+    """cadrPropagateCullBatch (every job's leaf level fused with the batched
+    cull) against the oracle: leaf world matrices, spheres where requested,
+    commands and counts bit for bit, for each supported shape, twice per
+    shape (the second launch reuses the status words under a new epoch)."""
+    from cadr_amd.device import synchronize
+    cases = _fused_batch_cases()
+    entries, exp, leaves = [], [], []
+    for j, sc in enumerate(cases):
+        worlds, sph = oracle.propagate_scene(sc)
+        views = cull.make_views(sc.cameras)
+        ec, en = oracle.cull(sph, sc.mesh_index, sc.meshes, sc.stateset_counts, views, cmd_dtype=N.CMD_DTYPE,
+                             nthreads=8)
+        ds = cull.DeviceScene(sc)
+        descs = ds.level_descs(spheres=j % 2 == 0)  # spheres requested for every other job
+        for lv in descs[:-1]:
+            N.check(N.load().cadrPropagateTransforms(N.ptr(lv), None))
+        leaves.append(descs[-1])
+        entries.append((ds, views, cull.CullOutputs(views.shape[0], ds.n, ds.K)))
+        exp.append((worlds[-1], sph, ec, en))
+    assert sum(int(e[3].sum()) for e in exp) > 1000
+    leaves = np.concatenate(leaves)
+    for block, items in ((256, 4), (256, 8), (512, 4)):
+        b = cull.CullBatch(cull.make_config(block, items), entries)
+        b.set_leaves(leaves)
+        for rep in range(2):
+            print(f"fused batch {block}x{items} rep {rep}", flush=True)
+            for ds, _, _ in entries:
+                ds.spheres.upload(np.full((ds.n, 4), np.nan, np.float32))
+                ds.levels[-1][3].upload(np.full((ds.n, 16), np.nan, np.float32))
+            b.run_fused_batch()
+            synchronize()
+            for j, ((ds, views, out), (world, sph, ec, en)) in enumerate(zip(entries, exp)):
+                what = f"{ds.scene.name} {block}x{items} rep {rep}"
+                cmds, counts = out.download()
+                _compare(cmds, counts, ec, en, ds.stateset_counts, what)
+                assert ds.world_matrices(-1).tobytes() == world.tobytes(), f"{what}: world matrices"
+                got = ds.download_spheres()
+                if j % 2 == 0:
+                    assert got.tobytes() == sph.tobytes(), f"{what}: spheres"
+                else:
+                    assert np.isnan(got).all(), f"{what}: spheres written although not requested"
+        b.close()
+    # unsupported shapes and a leaf that does not match its job are refused
+    b = cull.CullBatch(cull.make_config(256, 4), entries)
+    b.set_leaves(leaves)
+    for cfg in (cull.make_config(256, 2), cull.make_config(128, 8)):
+        b.config = cfg
+        with pytest.raises(N.CadrError):
+            b.run_fused_batch()
+    b.config = cull.make_config(256, 4)
+    bad = leaves.copy()
+    bad["count"][1] += 1
+    b.set_leaves(bad)
+    with pytest.raises(N.CadrError):
+        b.run_fused_batch()
+    bad = leaves.copy()
+    bad["parentIndex"][2] = 0
+    b.set_leaves(bad)
+    with pytest.raises(N.CadrError):
+        b.run_fused_batch()
+    b.close()
+    for ds, _, out in entries:
+        out.close(); ds.close()
