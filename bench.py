@@ -79,6 +79,7 @@ def parse():
     p.add_argument("--no-served", action="store_true")
     p.add_argument("--no-gather", action="store_true")
     p.add_argument("--no-resident", action="store_true")
+    p.add_argument("--no-hbm-probe", action="store_true", help="skip the measured copy / read / write bandwidth")
     p.add_argument("--resident-drawables", type=int, default=10_000_000)
     p.add_argument("--served-rates", default="5000,10000,20000,50000", help="Poisson arrival rates (req/s)")
     p.add_argument("--served-seconds", type=float, default=1.0, help="arrival window per rate")
@@ -300,12 +301,31 @@ def main():
 
     if not args.no_resident:
         result["resident"] = bench_resident(args)
+    if not args.no_hbm_probe:
+        result["hbm_probe"] = hbm_probe()
+        result["roofline"]["frac_of_measured_copy"] = fused_gbs / result["hbm_probe"]["copy_gbs"]
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, stream, lib, N, device)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)
+
+
+def hbm_probe():
+    """Measured HBM ceiling of this GPU (cadr_amd/bin/hbm_probe, HIP): a
+    streaming copy, read-only and write-only pass over 1 GiB, GB/s."""
+    import subprocess
+    exe = os.path.join(ROOT, "cadr_amd", "bin", "hbm_probe")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is not built (make -C cadr_amd/csrc)")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    if p.returncode != 0:
+        raise RuntimeError(f"hbm_probe failed ({p.returncode}): {p.stderr[-2000:]}")
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    r["what"] = "one float4 per thread over 1 GiB, 20 launches after 2 warm-up; copy counts read + write bytes"
+    log(f"hbm probe: copy {r['copy_gbs']:.0f} GB/s, read {r['read_gbs']:.0f} GB/s, write {r['write_gbs']:.0f} GB/s")
+    return r
 
 
 def bench_resident(args, frames=10):
