@@ -429,8 +429,9 @@ def test_root_and_child_level_in_one_launch(gpu):
 def _fused_batch_cases():
     """Scenes for the fused batched launch: tree depths 1 (flat: the leaf is
     a root level) to 4, LOD counts 1 to 4, 1 to 65 views (two view groups),
-    a mesh table too large for LDS, and a scene with invalid mesh indices and
-    parent indices past the parent level (those nodes act as roots)."""
+    a mesh table too large for LDS, a scene with invalid mesh indices and
+    parent indices past the parent level (those nodes act as roots), and two
+    scenes without views."""
     out = []
     for i, (n, depth, lods, nv) in enumerate([(3000, 1, 1, 3), (50_000, 2, 4, 17), (90_000, 3, 2, 1), (1, 1, 1, 5),
                                               (40_000, 4, 3, 65)]):
@@ -449,6 +450,9 @@ def _fused_batch_cases():
     sc.mesh_index = mi
     sc.levels[1] = (local, parent)
     out.append(sc)
+    # jobs without views still propagate their leaf level (spheres off, on)
+    out.append(scenes.uneven_scene(71, 5000, 2, 2, 0))
+    out.append(scenes.uneven_scene(72, 7000, 3, 1, 0))
     return out
 
 
