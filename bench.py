@@ -42,20 +42,39 @@ HBM_PEAK_GBS = 8000.0
 PROFILES = os.path.join(ROOT, "profiles")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` as measured by rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration) and
-    recorded in a profiles/*.md summary as a line
-    `traffic_bytes_per_launch <kernel> <bytes>`; None if not recorded."""
+def measured_per_launch(what: str, kernel: str):
+    """A per-launch figure of `kernel` measured with rocprofv3 --pmc and
+    recorded in a profiles/*.md summary as a line `<what>_per_launch
+    <kernel> <value>`; None if not recorded."""
     import glob
     import re
     for path in sorted(glob.glob(os.path.join(PROFILES, "*.md"))):
         with open(path) as f:
             for line in f:
-                m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+(\d+)", line)
+                m = re.match(r"\s*" + re.escape(what) + r"_per_launch\s+(\S+)\s+(\d+)", line)
                 if m and m.group(1) == kernel:
                     return int(m.group(2))
     return None
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
+    calibration), from a `traffic_bytes_per_launch` line."""
+    return measured_per_launch("traffic_bytes", kernel)
+
+
+def valu_issue(kernel: str, ms: float, cus: int, clock_ghz: float = 2.4):
+    """VALU issue rate of a launch: SQ_INSTS_VALU wave-instructions per
+    launch (a `valu_wave_instructions_per_launch` line) against one wave64
+    VALU instruction per CU per clock (4 SIMD16 units, 4 cycles each), at
+    the peak engine clock; None if not recorded."""
+    n = measured_per_launch("valu_wave_instructions", kernel)
+    if n is None:
+        return None
+    cap = cus * clock_ghz * 1e9 * ms * 1e-3
+    return {"wave_instructions": n, "issue_frac": n / cap, "clock_ghz": clock_ghz, "cus": cus,
+            "lane_ops_per_s": n * 64 / (ms * 1e-3),
+            "note": "no packed FP32 (disabled at compile time), so the non-packed lane rate is the ceiling"}
 
 
 def log(*a):
@@ -312,6 +331,11 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+def cu_count(lib):
+    n = ctypes.c_int(0)
+    return n.value if lib.cadrGetComputeUnitCount(ctypes.byref(n)) == 0 and n.value > 0 else 256
+
+
 def hbm_probe():
     """Measured HBM ceiling of this GPU (cadr_amd/bin/hbm_probe, HIP): a
     streaming copy, read-only and write-only pass over 1 GiB, GB/s."""
@@ -426,7 +450,9 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device, p
         "roofline": {"bound": "hbm", "kernel": f"cullKernel<{config.blockSize},{config.itemsPerThread}>",
                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"cullKernel<{config.blockSize},{config.itemsPerThread}>"),
-                     "algorithmic_bytes": alg_bytes},
+                     "algorithmic_bytes": alg_bytes,
+                     "valu": valu_issue(f"cullKernel<{config.blockSize},{config.itemsPerThread}>", batch_kernel_ms,
+                                        cu_count(lib))},
         "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags,
                    "autotune": tuned},
         "plain_config": {"block": view_config.blockSize, "items_per_thread": view_config.itemsPerThread},
