@@ -106,3 +106,43 @@ def test_result_strings():
     lib = N.load()
     assert lib.cadrGetResultString(0) == b"CADR_SUCCESS"
     assert lib.cadrGetResultString(2) == b"CADR_ERROR_OUT_OF_RESOURCES"
+
+
+def test_fused_batch_argument_checks_before_any_device_call():
+    """cadrPropagateCullBatch refuses unsupported shapes, missing arrays and
+    leaf levels that do not match their jobs on the host, before it touches
+    the device (the addresses below are never dereferenced)."""
+    lib = N.load()
+    fake = 1 << 40  # a 16-byte aligned address the checks only inspect
+    jobs = np.zeros(2, dtype=N.JOB_DTYPE)
+    jobs["numDrawables"] = (100, 7)
+    leaves = np.zeros(2, dtype=N.LEVEL_DTYPE)
+    leaves["count"] = (100, 7)
+    leaves["localMatrices"] = leaves["worldMatrices"] = fake
+
+    def call(cfg, dl=fake, hl=leaves, hj=jobs, n=2, ws=None):
+        return lib.cadrPropagateCullBatch(dl, None if hl is None else N.ptr(hl), ctypes.byref(cfg), fake,
+                                          None if hj is None else N.ptr(hj), n, ws, None)
+
+    ok = cull.make_config(256, 8)
+    for block, items in ((256, 2), (256, 16), (512, 2), (128, 8), (1024, 1), (256, 3)):
+        assert call(cull.make_config(block, items)) == N.ERROR_INVALID_VALUE, (block, items)
+    assert call(ok, n=0) == 0  # nothing to do
+    assert call(ok, dl=0) == N.ERROR_INVALID_VALUE
+    assert call(ok, hl=None) == N.ERROR_INVALID_VALUE
+    assert call(ok, hj=None) == N.ERROR_INVALID_VALUE
+    assert call(ok, dl=fake + 4) == N.ERROR_INVALID_VALUE  # misaligned descriptors
+    bad = leaves.copy(); bad["count"][1] = 8
+    assert call(ok, hl=bad) == N.ERROR_INVALID_VALUE
+    assert "leaf level must hold" in lib.cadrGetLastErrorMessage().decode()
+    bad = leaves.copy(); bad["worldMatrices"][0] = 0
+    assert call(ok, hl=bad) == N.ERROR_INVALID_VALUE
+    bad = leaves.copy(); bad["localMatrices"][1] = fake + 8
+    assert call(ok, hl=bad) == N.ERROR_INVALID_VALUE
+    bad = leaves.copy(); bad["parentWorld"][0] = fake  # a non-root level needs parent indices
+    assert call(ok, hl=bad) == N.ERROR_INVALID_VALUE
+    bad["parentIndex"][0] = fake + 2
+    assert call(ok, hl=bad) == N.ERROR_INVALID_VALUE
+    # well-formed leaves reach the workspace check
+    assert call(ok) == N.ERROR_INVALID_VALUE
+    assert "workspace is NULL" in lib.cadrGetLastErrorMessage().decode()
