@@ -507,11 +507,29 @@ def batched_frames(args, entries, leaf_descs, propagate_inner, batch, stream, li
         same = same and all(np.array_equal(e[2].download()[1], c) for e, c in zip(entries, counts_un))
         b.close()
     best = min(sweep, key=sweep.get)
+    # the fused launch alone (the levels above are already propagated), for its roofline
+    b = cull.CullBatch(cull.make_config(*(int(x) for x in best.split("x"))), entries)
+    b.set_leaves(fused_leaves)
+    fused_ms = time_graph(lambda: b.run_fused_batch(s), stream, steps)
+    b.close()
+    # per leaf: 64-B local + 4-B parent index + 4-B mesh index read, 64-B world
+    # written; each parent's world matrix read once; 20 B per visible command
+    # and 4 B per (view, StateSet) count written
+    leaves_n = int(leaves["count"].sum())
+    parents_n = int(leaves["parentCount"][leaves["parentWorld"] != 0].sum())
+    survivors = int(sum(c.sum() for c in counts_un))
+    fused_bytes = (leaves_n * (64 + 4 + 4 + 64) + parents_n * 64 + survivors * 20 +
+                   sum(e[0].K * e[1].shape[0] * 4 for e in entries))
+    fused_gbs = fused_bytes / (fused_ms * 1e-3) / 1e9
     r = {"unfused_ms_per_frame": un_ms, "fused_ms_per_frame": sweep[best], "fused_shape": best,
          "fused_candidates_ms": sweep, "speedup_fused": un_ms / sweep[best], "fused_counts_identical": bool(same),
+         "fused_roofline": {"bound": "hbm", "kernel": f"cullKernel<{best.replace('x', ',')},true>",
+                            "kernel_ms": fused_ms, "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": fused_gbs / HBM_PEAK_GBS, "algorithmic_bytes": fused_bytes},
          "frame": "levels above the leaves + (leaf propagation launch(es), batched cull) or the leaf levels fused "
                   "into the batched launch; one HIP graph per frame"}
-    log(f"{name} frame: unfused {un_ms:.3f} ms, fused {sweep[best]:.3f} ms ({best}) {sweep}, identical={same}")
+    log(f"{name} frame: unfused {un_ms:.3f} ms, fused {sweep[best]:.3f} ms ({best}) {sweep}, identical={same}; "
+        f"fused kernel {fused_ms:.3f} ms ({fused_gbs:.0f} GB/s alg)")
     return r
 
 
