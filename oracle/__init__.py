@@ -42,6 +42,8 @@ def lib() -> ctypes.CDLL:
             "oracle_gltf_trs_matrix": (None, [_P, _P, _P, _P]),
             "oracle_set_threads": (None, [ctypes.c_int]),
             "oracle_gltf_root_flip": (None, [_P, _P]),
+            "oracle_gltf_primitive_sphere": (None, [_P, _u64, _P, _P, _P]),
+            "oracle_gltf_mesh_sphere": (None, [_P, _P, _u64, _P]),
             "oracle_propagate_level": (None, [_P, _P, _u32, _P, _P, _u64, _P, _P, _u32, _P]),
             "oracle_make_view": (_i, [_P, _P, _P, _f, _f, _f, _f, _f, _P, _P]),
             "oracle_test_drawable": (_i, [_P, _P, _P]),
@@ -71,6 +73,8 @@ def ref() -> ctypes.CDLL:
             "ref_propagate_tree": (None, [_P, _P, _u64, _P]),
             "ref_gltf_trs_matrix": (None, [_P, _P, _P, _P]),
             "ref_gltf_root_flip": (None, [_P, _P]),
+            "ref_gltf_primitive_sphere": (None, [_P, _u64, _P, _P, _P]),
+            "ref_gltf_mesh_sphere": (None, [_P, _P, _u64, _P]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -111,6 +115,28 @@ def gltf_trs_matrix(scale, rotation=None, translation=None, ref=False):
     f = globals()["ref"]().ref_gltf_trs_matrix if ref else lib().oracle_gltf_trs_matrix
     f(_p(s), _p(q), _p(t), _p(M))
     return M
+
+
+def gltf_primitive_sphere(positions, box_min, box_max, ref=False):
+    """glTF primitive bounding sphere {centre, radius} (main.cpp:3196-3245,
+    3597-3600); ref=True runs the reference's BoundingSphere code."""
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    out = np.zeros(4, dtype=np.float32)
+    f = globals()["ref"]().ref_gltf_primitive_sphere if ref else lib().oracle_gltf_primitive_sphere
+    f(_p(p), p.shape[0], _p(np.ascontiguousarray(box_min, np.float32)), _p(np.ascontiguousarray(box_max, np.float32)),
+      _p(out))
+    return out
+
+
+def gltf_mesh_sphere(boxes, spheres, ref=False):
+    """glTF mesh bounding sphere from its primitives' boxes (k x 6) and
+    spheres (k x 4) (main.cpp:2714, 3195, 3762-3769)."""
+    b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+    s = np.ascontiguousarray(spheres, np.float32).reshape(-1, 4)
+    out = np.zeros(4, dtype=np.float32)
+    f = globals()["ref"]().ref_gltf_mesh_sphere if ref else lib().oracle_gltf_mesh_sphere
+    f(_p(b), _p(s), b.shape[0], _p(out))
+    return out
 
 
 def gltf_root_flip(M, ref=False):

@@ -189,6 +189,63 @@ void oracle_gltf_root_flip(const float* M, float* out)
 	}
 }
 
+/* glm::distance2(p0, p1) = length2(p1 - p0) = (d.x*d.x + d.y*d.y) + d.z*d.z
+ * (gtx/norm.inl:9-37, detail/func_geometric.inl dot of vec3). */
+static float distance2(const float* p0, const float* p1)
+{
+	const float dx = p1[0] - p0[0], dy = p1[1] - p0[1], dz = p1[2] - p0[2];
+	return (dx * dx + dy * dy) + dz * dz;
+}
+
+/* Primitive sphere, main.cpp:3196-3245 and :3597-3600: centre = the POSITION
+ * accessor box's getCenter() = (min + max) / 2 (BoundingBox.h:44); radius
+ * holds radius^2 while each position p is added by
+ * extendRadiusByPointUsingRadius2 (BoundingSphere.h:65-69: d2 =
+ * distance2(p, centre), kept when d2 > radius, so NaN never is), then
+ * radius = sqrt(radius) with main.cpp's `using namespace std`: the float
+ * overload. */
+void oracle_gltf_primitive_sphere(const float* positions, uint64_t n, const float* boxMin, const float* boxMax,
+                                  float* out)
+{
+	float c[3];
+	for (int a = 0; a < 3; a++) c[a] = (boxMin[a] + boxMax[a]) / 2.0f;
+	float r2 = 0.0f;
+	for (uint64_t i = 0; i < n; i++) {
+		const float d2 = distance2(positions + 3 * i, c);
+		if (d2 > r2) r2 = d2;
+	}
+	out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+	out[3] = sqrtf(r2);
+}
+
+/* Mesh sphere, main.cpp:2714, :3195, :3762-3769: the union of the
+ * primitives' boxes (BoundingBox::empty() then extendBy, BoundingBox.h:
+ * 30-35, 55-68: a component is replaced only by a smaller min / larger max),
+ * centred there with radius 0 and grown by each primitive sphere in order
+ * (extendRadiusBy, BoundingSphere.h:50-58: d2 = distance2(centre, s.centre),
+ * dif = radius - s.radius, when dif < 0 or d2 > dif*dif the radius becomes
+ * sqrt(d2) + s.radius, that unqualified sqrt inside namespace CadR being
+ * ::sqrt(double), so the sum is rounded to float once). */
+void oracle_gltf_mesh_sphere(const float* boxes, const float* spheres, uint64_t k, float* out)
+{
+	float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+	for (uint64_t i = 0; i < k; i++)
+		for (int a = 0; a < 3; a++) {
+			if (boxes[6 * i + a] < mn[a]) mn[a] = boxes[6 * i + a];
+			if (boxes[6 * i + 3 + a] > mx[a]) mx[a] = boxes[6 * i + 3 + a];
+		}
+	float c[3];
+	for (int a = 0; a < 3; a++) c[a] = (mn[a] + mx[a]) / 2.0f;
+	float r = 0.0f;
+	for (uint64_t i = 0; i < k; i++) {
+		const float d2 = distance2(c, spheres + 4 * i);
+		const float dif = r - spheres[4 * i + 3];
+		if (dif < 0.0f || d2 > dif * dif) r = (float)(sqrt((double)d2) + (double)spheres[4 * i + 3]);
+	}
+	out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+	out[3] = r;
+}
+
 void oracle_sphere_transform(const float* M, const float* s, float* out)
 {
 	float c[3];

@@ -50,6 +50,12 @@ void oracle_sphere_transform(const float* M, const float* sphere, float* out);
  * none) and the root-node axis flip (examples/gltfReader/main.cpp:1455-1553). */
 void oracle_gltf_trs_matrix(const float* s, const float* q, const float* t, float* M);
 void oracle_gltf_root_flip(const float* M, float* out);
+/* glTF primitive and mesh bounding spheres (examples/gltfReader/main.cpp:
+ * 2714, 3191-3245, 3597-3600, 3762-3769): out = {centre xyz, radius}.
+ * positions: n x 3; boxes: k x {min xyz, max xyz}; spheres: k x 4. */
+void oracle_gltf_primitive_sphere(const float* positions, uint64_t n, const float* boxMin, const float* boxMax,
+                                  float* out);
+void oracle_gltf_mesh_sphere(const float* boxes, const float* spheres, uint64_t k, float* out);
 /* Threads used by oracle_propagate_level (default 1). */
 void oracle_set_threads(int n);
 /* One level: world = parentWorld[parent[i]] * local[i]; parentWorld == NULL

@@ -105,6 +105,34 @@ void ref_gltf_trs_matrix(const float* s, const float* q, const float* t, float* 
 }
 
 // Root matrix of a glTF scene (main.cpp:1547-1553): y and z rows negated.
+// glTF primitive sphere as examples/gltfReader/main.cpp:3196-3245, 3597-3600
+// builds it with the reference's BoundingBox / BoundingSphere (main.cpp has
+// `using namespace std`, so its sqrt of a float is std::sqrt(float)).
+void ref_gltf_primitive_sphere(const float* positions, uint64_t n, const float* boxMin, const float* boxMax,
+                               float* out)
+{
+	CadR::BoundingBox bb{glm::vec3(boxMin[0], boxMin[1], boxMin[2]), glm::vec3(boxMax[0], boxMax[1], boxMax[2])};
+	CadR::BoundingSphere bs{bb.getCenter(), 0.f};
+	for (uint64_t i = 0; i < n; i++)
+		bs.extendRadiusByPointUsingRadius2(glm::vec3(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2]));
+	bs.radius = std::sqrt(bs.radius);
+	out[0] = bs.center.x; out[1] = bs.center.y; out[2] = bs.center.z; out[3] = bs.radius;
+}
+
+// glTF mesh sphere as main.cpp:2714, 3195, 3762-3769 builds it.
+void ref_gltf_mesh_sphere(const float* boxes, const float* spheres, uint64_t k, float* out)
+{
+	CadR::BoundingBox bb = CadR::BoundingBox::empty();
+	for (uint64_t i = 0; i < k; i++)
+		bb.extendBy(CadR::BoundingBox{glm::vec3(boxes[6 * i], boxes[6 * i + 1], boxes[6 * i + 2]),
+		                              glm::vec3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5])});
+	CadR::BoundingSphere bs{bb.getCenter(), 0.f};
+	for (uint64_t i = 0; i < k; i++)
+		bs.extendRadiusBy(CadR::BoundingSphere{glm::vec3(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]),
+		                                       spheres[4 * i + 3]});
+	out[0] = bs.center.x; out[1] = bs.center.y; out[2] = bs.center.z; out[3] = bs.radius;
+}
+
 void ref_gltf_root_flip(const float* M, float* out)
 {
 	glm::mat4 m = loadMat(M);
