@@ -520,3 +520,50 @@ def test_fused_batch_propagation_and_cull(gpu):
     b.close()
     for ds, _, out in entries:
         out.close(); ds.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_cull_batched_random_scenes(gpu, seed):
+    """Random scenes against the oracle: 1-80 StateSets (some empty), 1-130
+    views (up to three view groups), mesh tables of 1-100 meshes with 1-4
+    LODs and invalid mesh indices, and irregular spheres (NaN components,
+    infinite, negative and empty radii) that switch the tile sphere off for
+    their tiles; two batched shapes per seed, command stride past the
+    drawable count."""
+    from cadr_amd.device import synchronize
+    rng = np.random.default_rng(200 + seed)
+    K = int(rng.integers(1, 81))
+    counts_in = rng.integers(0, 3000, K).astype(np.uint32)
+    counts_in[rng.random(K) < 0.2] = 0
+    n = int(counts_in.sum())
+    c = rng.uniform(-200, 200, (n, 3)).astype(np.float32)
+    r = rng.uniform(0, 20, (n, 1)).astype(np.float32)
+    s = np.concatenate([c, r], axis=1).astype(np.float32)
+    for frac, val, col in ((0.002, np.nan, 0), (0.002, np.inf, 3), (0.002, -1.0, 3), (0.002, -np.inf, 3)):
+        s[rng.random(n) < frac, col] = val
+    nm = int(rng.integers(1, 101))
+    meshes = scenes.make_meshes(300 + seed, nm, rng.integers(1, 5, nm))
+    mi = rng.integers(0, nm + 2, n).astype(np.uint32)  # a few invalid mesh indices
+    V = int(rng.integers(1, 131))
+    cams = scenes.random_cameras(scenes.SEED_BASE + 50 + seed, 20, 0, V, box=250.0)
+    views = cull.make_views(cams)
+    exp_cmds, exp_counts = oracle.cull(s, mi, meshes, counts_in, views, cmd_dtype=N.CMD_DTYPE, nthreads=8)
+    sc = scenes.Scene(f"random{seed}", [(np.zeros((max(n, 1), 16), np.float32), None)], mi, meshes, counts_in, cams)
+    ds = cull.DeviceScene(sc, keep_levels=False)
+    if n:
+        ds.upload_spheres(s)
+    stride = max(n, 1) + int(rng.integers(0, 5))
+    shapes = [(256, 2), (256, 4), (256, 8), (256, 16), (512, 2), (512, 4), (128, 8), (128, 16)]
+    for i in rng.choice(len(shapes), 2, replace=False):
+        block, items = shapes[int(i)]
+        print(f"random seed {seed}: {n} drawables, {K} StateSets, {V} views, {nm} meshes, {block}x{items}", flush=True)
+        out = cull.CullOutputs(V, stride, K)
+        b = cull.CullBatch(cull.make_config(block, items), [(ds, views, out)])
+        b.run()
+        synchronize()
+        cmds, counts = out.download()
+        exp_full = np.zeros((V, stride), N.CMD_DTYPE)
+        exp_full[:, :n] = exp_cmds[:, :n]
+        _compare(cmds, counts, exp_full, exp_counts, counts_in, f"seed {seed} {block}x{items}")
+        b.close(); out.close()
+    ds.close()
