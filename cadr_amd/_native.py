@@ -167,6 +167,8 @@ SIGNATURES = {
     "cadrCullView": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_void_p, c_void_p]),
     "cadrCullTileSize": (c_u32, [ctypes.POINTER(CullConfig)]),
     "cadrPropagateCullView": (c_int, [c_void_p, ctypes.POINTER(CullConfig), c_u64, c_void_p, c_void_p, c_void_p]),
+    "cadrCompactDrawablePointers": (c_int, [c_u64, c_u64, c_u64, c_u32, c_u32, c_u32, c_u64, c_u64, c_u64,
+                                            c_void_p]),
     "cadrPropagateCullBatch": (c_int, [c_u64, c_void_p, ctypes.POINTER(CullConfig), c_u64, c_void_p, c_u32,
                                        c_void_p, c_void_p]),
     "cadrServerCreate": (c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(c_void_p)]),

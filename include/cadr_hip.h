@@ -188,6 +188,25 @@ int cadrProcessDrawables(const cadrProcessDrawablesPushConstants* pushConstants,
  * Renderer.cpp:635-644 staging -> device copy) by the records that changed;
  * the facade's Renderer uses it when the drawable order is unchanged.  The
  * order of writes to one slot listed twice is unspecified. */
+/* DrawablePointers for the cull's compacted draws, the reference-layout
+ * companion of its commands.  The reference's shaders read
+ * DrawablePointersRef[gl_DrawID] (CadPL/UberShaderPoints.vert:38) from a
+ * per-StateSet base pushed before each StateSet's draw
+ * (StateSet.cpp:240-258), with records as processDrawables.comp:52-59 writes
+ * them.  For view w and StateSet k (base_k = stateSetBase[k], n = counts[w*K
+ * + k]): pointersOut[w*commandStride + base_k + j] =
+ * pointersIn[commands[w*commandStride + base_k + j].firstInstance] for
+ * j < n (zeros for an index at or past numDrawables), so drawing StateSet k
+ * of view w with vkCmdDrawIndexedIndirectCount from slot base_k and
+ * pushing pointersOut + (w*commandStride + base_k)*32 as the StateSet's
+ * pointer base gives every draw its own drawable's record.  Slots past a
+ * segment's count are left unwritten.  commands / counts as a cull job
+ * writes them, stateSetBase uint32[K] (device), pointersIn / pointersOut
+ * 32-byte records (16-byte aligned, device). */
+int cadrCompactDrawablePointers(uint64_t commands, uint64_t counts, uint64_t stateSetBase, uint32_t numViews,
+                                uint32_t numStateSets, uint32_t commandStride, uint64_t pointersIn,
+                                uint64_t numDrawables, uint64_t pointersOut, cadrStream stream);
+
 int cadrScatterDrawables(uint64_t drawableList, uint64_t listSlots, uint64_t records, uint64_t slots,
                          uint32_t count, cadrStream stream);
 

@@ -152,3 +152,71 @@ def test_gather_scene_generator_matches_oracle():
         assert rc == 0
         ei, ep = g.expected(base)
         assert np.array_equal(ind, ei) and np.array_equal(ptr, ep), f"n={n} h0={h0}"
+
+
+@pytest.mark.gpu
+def test_gpu_compacted_drawable_pointers(gpu):
+    """The reference-layout companion of the cull: the gather writes every
+    drawable's DrawablePointers, the cull compacts the visible ones per
+    (view, StateSet), and cadrCompactDrawablePointers lays the records out
+    beside the commands, so DrawablePointers[gl_DrawID] of a StateSet's draw
+    is its own drawable's record.  Checked against the gather's closed form
+    and the oracle's command lists; slots past each count stay untouched."""
+    from cadr_amd import cull, scenes
+    from cadr_amd.device import DeviceBuffer, synchronize
+    n, K = 60_000, 9
+    g = scenes.GatherScene(n, first_handle=1)
+    buf = DeviceBuffer(g.layout(0)["size"])
+    img, push, level = g.image(buf.address)
+    buf.upload(img)
+    ind, ptr = DeviceBuffer(16 * n), DeviceBuffer(32 * n)
+    pc = N.PushConstants(push[0], push[1], ind.address, ptr.address)
+    N.check(N.load().cadrProcessDrawables(ctypes.byref(pc), level, n, None))
+    _, records = g.expected(buf.address)  # uint64 [n, 4], the gather's closed form
+
+    sc = scenes.plain_scene(n, seed=scenes.SEED_BASE + 61, num_statesets=K)
+    sc.mesh_index[::97] = 1000  # invalid meshes: never visible
+    cams = scenes.random_cameras(scenes.SEED_BASE + 62, 20, 0, 5, box=300.0) + [scenes.DEFAULT_CAMERA]
+    views = cull.make_views(cams)
+    _, sph = oracle.propagate_scene(sc)
+    exp_cmds, exp_counts = oracle.cull(sph, sc.mesh_index, sc.meshes, sc.stateset_counts, views, cmd_dtype=N.CMD_DTYPE)
+    assert exp_counts.sum() > 1000
+    ds = cull.DeviceScene(sc)
+    ds.propagate()
+    V, stride = views.shape[0], n + 3
+    out = cull.CullOutputs(V, stride, K)
+    b = cull.CullBatch(cull.make_config(256, 8), [(ds, views, out)])
+    b.run()
+    base = np.concatenate([[0], np.cumsum(sc.stateset_counts)[:-1]]).astype(np.uint32)
+    bb = DeviceBuffer.from_array(base)
+    po = DeviceBuffer(32 * V * stride)
+    po.fill_u32(0xA5A5A5A5)
+    N.check(N.load().cadrCompactDrawablePointers(out.commands.address, out.counts.address, bb.address, V, K, stride,
+                                                 ptr.address, n, po.address, None))
+    synchronize()
+    cmds, counts = out.download()
+    assert np.array_equal(counts, exp_counts)
+    got = po.download(np.uint64, 4 * V * stride).reshape(V, stride, 4)
+    sentinel = np.uint64(0xA5A5A5A5A5A5A5A5)
+    for w in range(V):
+        for k in range(K):
+            b0, c = int(base[k]), int(counts[w, k])
+            seg = exp_cmds[w, b0:b0 + c]
+            assert np.array_equal(cmds[w, b0:b0 + c], seg)
+            # the records of the visible drawables, in the order of their commands
+            assert np.array_equal(got[w, b0:b0 + c], records[seg["firstInstance"].astype(np.int64)]), (w, k)
+            end = b0 + int(sc.stateset_counts[k])
+            assert np.all(got[w, b0 + c:end] == sentinel), (w, k)
+    assert np.all(got[:, n:] == sentinel)
+    # an index past the drawable count gives a zero record
+    bad = np.zeros(1, N.CMD_DTYPE)
+    bad["firstInstance"] = n + 5
+    c1 = DeviceBuffer.from_array(np.concatenate([bad, np.zeros(stride - 1, N.CMD_DTYPE)]))
+    cnt = DeviceBuffer.from_array(np.array([1] + [0] * (K - 1), np.uint32))
+    N.check(N.load().cadrCompactDrawablePointers(c1.address, cnt.address, bb.address, 1, K, stride, ptr.address, n,
+                                                 po.address, None))
+    synchronize()
+    assert np.all(po.download(np.uint64, 4) == 0)
+    for x in (buf, ind, ptr, bb, po, c1, cnt, out, ds):
+        x.close()
+    b.close()
