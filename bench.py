@@ -301,6 +301,28 @@ def main():
                           "cull_gbs": cull_bytes / (cull_ms * 1e-3) / 1e9,
                           "cull_config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread}},
     }
+    # reference layout: the gather's DrawablePointers of the visible drawables
+    # beside their commands (cadrCompactDrawablePointers), a synthetic
+    # 32-B record per drawable as its input
+    ptr_in = device.DeviceBuffer(32 * n)
+    N.check(lib.cadrFillU32(ptr_in.address, 0x5A5A5A5A, 32 * n, s))
+    ptr_out = device.DeviceBuffer(32 * n)
+    ss_base = device.DeviceBuffer.from_array(
+        np.concatenate([[0], np.cumsum(ds.stateset_counts)[:-1]]).astype(np.uint32))
+
+    def compact_pointers():
+        N.check(lib.cadrCompactDrawablePointers(out1.commands.address, out1.counts.address, ss_base.address, 1, ds.K,
+                                                out1.stride, ptr_in.address, n, ptr_out.address, s))
+
+    ptr_ms = time_graph(compact_pointers, stream, max(args.steps, 3))
+    ptr_bytes = survivors1 * (4 + 32 + 32) + ds.K * 4
+    result["reference_layout"] = {
+        "what": "cadrCompactDrawablePointers after the frame: each visible drawable's 32-B DrawablePointers "
+                "beside its command (the gl_DrawID contract of the reference's shaders)",
+        "pointers_ms": ptr_ms, "ms_per_frame": ms_per_step + ptr_ms,
+        "pointers_gbs": ptr_bytes / (ptr_ms * 1e-3) / 1e9, "algorithmic_bytes": ptr_bytes}
+    for x in (ptr_in, ptr_out, ss_base):
+        x.close()
     log(f"full10m fused: {ms_per_step:.3f} ms/frame, fused kernel {fused_ms:.3f} ms ({fused_gbs:.0f} GB/s); "
         f"unfused {un_ms:.3f} ms/frame (leaf {leaf_ms:.3f} ms {leaf_gbs:.0f} GB/s, cull {cull_ms:.3f} ms); "
         f"visible {survivors1}, identical={same}")

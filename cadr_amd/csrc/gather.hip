@@ -119,23 +119,27 @@ __global__ __launch_bounds__(256) void scatterDrawablesKernel(const uint4* __res
 	list[3 * slot + 2] = c;
 }
 
-// Compacted DrawablePointers of a cull output: one workgroup per (view,
-// StateSet) segment; slot j of the segment takes the 32-byte record of the
-// drawable its command names (firstInstance, dword 4 of the 20-byte
-// command), or zeros for an index past the drawable count.
+// Compacted DrawablePointers of a cull output: `split` workgroups per
+// (view, StateSet) segment, workgroup c taking slots c*256 + t, stepping by
+// split*256 (the counts are only known on the device, so every segment gets
+// the same number of workgroups, enough for one of average size); slot j of
+// the segment takes the 32-byte record of the drawable its command names
+// (firstInstance, dword 4 of the 20-byte command), or zeros for an index
+// past the drawable count.
 __global__ __launch_bounds__(256) void compactPointersKernel(const uint32_t* __restrict__ commands,
                                                              const uint32_t* __restrict__ counts,
                                                              const uint32_t* __restrict__ stateSetBase,
                                                              uint32_t numStateSets, uint32_t commandStride,
-                                                             const uint4* __restrict__ pointersIn,
+                                                             uint32_t split, const uint4* __restrict__ pointersIn,
                                                              uint64_t numDrawables, uint4* __restrict__ pointersOut)
 {
-	const uint32_t w = blockIdx.x / numStateSets, k = blockIdx.x % numStateSets;
+	const uint32_t seg = blockIdx.x / split, c = blockIdx.x % split;
+	const uint32_t w = seg / numStateSets, k = seg % numStateSets;
 	const uint32_t base = stateSetBase[k];
 	if (base >= commandStride) return;
 	const uint32_t n = min(counts[uint64_t(w) * numStateSets + k], commandStride - base);
 	const uint64_t first = uint64_t(w) * commandStride + base;
-	for (uint32_t j = threadIdx.x; j < n; j += 256u) {
+	for (uint32_t j = c * 256u + threadIdx.x; j < n; j += split * 256u) {
 		const uint64_t slot = first + j;
 		const uint32_t i = commands[5 * slot + 4];
 		uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
@@ -162,11 +166,15 @@ CADR_API int cadrCompactDrawablePointers(uint64_t commands, uint64_t counts, uin
 	CADR_REQUIRE((commands % 4) == 0 && (counts % 4) == 0 && (stateSetBase % 4) == 0 && (pointersIn % 16) == 0 &&
 	                 (pointersOut % 16) == 0,
 	             "cadrCompactDrawablePointers: pointer arrays must be 16-byte aligned, the others 4-byte aligned");
-	CADR_REQUIRE(uint64_t(numViews) * numStateSets < (1ull << 31),
+	// workgroups per segment: one per 2048 slots of an average segment
+	const uint64_t avg = (uint64_t(commandStride) + numStateSets - 1) / numStateSets;
+	const uint32_t split = uint32_t(std::max<uint64_t>(1, (avg + 2047) / 2048));
+	CADR_REQUIRE(uint64_t(numViews) * numStateSets * split < (1ull << 31),
 	             "cadrCompactDrawablePointers: too many (view, StateSet) segments");
-	hipLaunchKernelGGL(compactPointersKernel, dim3(numViews * numStateSets), dim3(256), 0, hipStream_t(stream),
-	                   reinterpret_cast<const uint32_t*>(commands), reinterpret_cast<const uint32_t*>(counts),
-	                   reinterpret_cast<const uint32_t*>(stateSetBase), numStateSets, commandStride,
+	hipLaunchKernelGGL(compactPointersKernel, dim3(numViews * numStateSets * split), dim3(256), 0,
+	                   hipStream_t(stream), reinterpret_cast<const uint32_t*>(commands),
+	                   reinterpret_cast<const uint32_t*>(counts), reinterpret_cast<const uint32_t*>(stateSetBase),
+	                   numStateSets, commandStride, split,
 	                   reinterpret_cast<const uint4*>(pointersIn), numDrawables, reinterpret_cast<uint4*>(pointersOut));
 	CADR_CHECK_HIP(hipGetLastError(), "compactPointersKernel launch");
 	return ok();
