@@ -140,8 +140,8 @@ def main():
     except OSError:
         pass
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; the pass shards by scene range or by view "
-                                   "(see DESIGN.md), and no multi-GPU figure is produced", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark measures one GPU only (the pass would shard by scene range "
+                                   "or by view); no multi-GPU figure is produced", "n_gpus": args.gpus}))
         sys.exit(2)
 
     from cadr_amd import _native as N
