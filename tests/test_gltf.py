@@ -1,6 +1,9 @@
-"""glTF hierarchy ingestion (examples/gltfReader/main.cpp:1422-1576): node
-matrices, the root axis flip, the depth-first walk flattened into levels, and
-GPU propagation against the reference's recursion restated on the oracle."""
+"""glTF ingestion (examples/gltfReader/main.cpp:1392-1576, 2700-3770): node
+matrices, the root axis flip, the depth-first walk flattened into levels, GPU
+propagation against the reference's recursion restated on the oracle; mesh
+data: buffers, accessors, index lists per primitive mode, primitive and mesh
+bounding spheres against the reference's BoundingSphere code, and a document
+culled end to end on the GPU against the oracle."""
 import json
 import struct
 
