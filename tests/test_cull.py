@@ -567,3 +567,56 @@ def test_cull_batched_random_scenes(gpu, seed):
         _compare(cmds, counts, exp_full, exp_counts, counts_in, f"seed {seed} {block}x{items}")
         b.close(); out.close()
     ds.close()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_fused_one_view_random_scenes(gpu, seed):
+    """Random trees through cadrPropagateCullView against the oracle: depth
+    1-4, 1-70 StateSets (some empty), parent indices past the parent level,
+    invalid mesh indices, a few non-finite local matrices (their tiles lose
+    the tile sphere), every fused shape; world matrices, commands and
+    counts bit for bit."""
+    from cadr_amd.device import synchronize
+    rng = np.random.default_rng(400 + seed)
+    depth = int(rng.integers(1, 5))
+    n = int(rng.integers(1, 120_000))
+    sc = scenes.uneven_scene(80 + seed, n, depth, int(rng.integers(1, 5)), 1, num_statesets=int(rng.integers(1, 71)))
+    local, parent = sc.levels[-1]
+    local = local.copy()
+    local[rng.random(n) < 0.001, int(rng.integers(0, 16))] = np.inf
+    if parent is not None:
+        parent = parent.copy()
+        parent[rng.random(n) < 0.01] = sc.levels[-2][0].shape[0] + 3
+    sc.levels[-1] = (local, parent)
+    mi = sc.mesh_index.copy()
+    mi[rng.random(n) < 0.01] = 64 + 7
+    sc.mesh_index = mi
+    e = rng.normal(size=3)
+    e = e / np.linalg.norm(e) * rng.uniform(120, 250)  # outside the scene, looking at its centre
+    d = -e / np.linalg.norm(e)
+    cam = scenes.Camera(eye=tuple(float(np.float32(x)) for x in e), dir=tuple(float(np.float32(x)) for x in d),
+                        up=(0.0, 1.0, 0.0) if abs(d[1]) < 0.99 else (1.0, 0.0, 0.0), fovy=float(rng.uniform(0.5, 1.5)))
+    views = cull.make_views([cam])
+    worlds, sph = oracle.propagate_scene(sc)
+    exp_cmds, exp_counts = oracle.cull(sph, sc.mesh_index, sc.meshes, sc.stateset_counts, views, cmd_dtype=N.CMD_DTYPE)
+    ds = cull.DeviceScene(sc)
+    descs = ds.level_descs(spheres=False)
+    for lv in descs[:-1]:
+        N.check(N.load().cadrPropagateTransforms(N.ptr(lv), None))
+    for block, items in ((256, 2), (256, 4), (256, 8), (256, 16), (512, 2), (512, 4), (1024, 1), (1024, 2)):
+        print(f"fused random seed {seed}: {n} drawables, depth {depth}, {ds.K} StateSets, {block}x{items}", flush=True)
+        ds.levels[-1][3].upload(np.full((n, 16), np.nan, np.float32))
+        out = cull.CullOutputs(1, n, ds.K)
+        b = cull.CullBatch(cull.make_config(block, items), [(ds, views, out)])
+        b.run_fused(descs[-1])
+        synchronize()
+        cmds, counts = out.download()
+        _compare(cmds, counts, exp_cmds, exp_counts, sc.stateset_counts, f"seed {seed} {block}x{items}")
+        # bit for bit except the payload of generated NaNs (inf * 0), which the
+        # host and the GPU encode differently
+        got, exp = ds.world_matrices(-1), worlds[-1]
+        nan = np.isnan(exp)
+        assert np.array_equal(np.isnan(got), nan)
+        assert np.array_equal(got.view(np.uint32)[~nan], exp.view(np.uint32)[~nan])
+        b.close(); out.close()
+    ds.close()
