@@ -620,3 +620,54 @@ def test_fused_one_view_random_scenes(gpu, seed):
         assert np.array_equal(got.view(np.uint32)[~nan], exp.view(np.uint32)[~nan])
         b.close(); out.close()
     ds.close()
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_fused_batch_random_jobs(gpu, seed):
+    """Random multi-scene launches of cadrPropagateCullBatch against the
+    oracle: 2-6 scenes of random depth, size and StateSet count, 0-70 views
+    each, out-of-range parents and invalid meshes; each fused shape."""
+    from cadr_amd.device import synchronize
+    rng = np.random.default_rng(500 + seed)
+    entries, exp, leaves = [], [], []
+    for j in range(int(rng.integers(2, 7))):
+        n = int(rng.integers(1, 60_000))
+        nv = int(rng.integers(0, 71))
+        sc = scenes.uneven_scene(100 + 10 * seed + j, n, int(rng.integers(1, 5)), int(rng.integers(1, 5)), nv,
+                                 num_statesets=int(rng.integers(1, 65)))
+        local, parent = sc.levels[-1]
+        if parent is not None:
+            parent = parent.copy()
+            parent[rng.random(n) < 0.01] = sc.levels[-2][0].shape[0] + 1
+            sc.levels[-1] = (local, parent)
+        mi = sc.mesh_index.copy()
+        mi[rng.random(n) < 0.01] = 99
+        sc.mesh_index = mi
+        sc.cameras = scenes.random_cameras(scenes.SEED_BASE + 120 + seed, 20 + j, 0, nv, box=250.0)
+        worlds, sph = oracle.propagate_scene(sc)
+        views = cull.make_views(sc.cameras)
+        ec, en = oracle.cull(sph, sc.mesh_index, sc.meshes, sc.stateset_counts, views, cmd_dtype=N.CMD_DTYPE,
+                             nthreads=8)
+        ds = cull.DeviceScene(sc)
+        descs = ds.level_descs(spheres=False)
+        for lv in descs[:-1]:
+            N.check(N.load().cadrPropagateTransforms(N.ptr(lv), None))
+        leaves.append(descs[-1])
+        entries.append((ds, views, cull.CullOutputs(nv, ds.n, ds.K)))
+        exp.append((worlds[-1], ec, en))
+    leaves = np.concatenate(leaves)
+    for block, items in ((256, 4), (256, 8), (512, 4)):
+        print(f"fused batch random seed {seed}: {len(entries)} scenes, {block}x{items}", flush=True)
+        for ds, _, _ in entries:
+            ds.levels[-1][3].upload(np.full((ds.n, 16), np.nan, np.float32))
+        b = cull.CullBatch(cull.make_config(block, items), entries)
+        b.set_leaves(leaves)
+        b.run_fused_batch()
+        synchronize()
+        for (ds, views, out), (world, ec, en) in zip(entries, exp):
+            cmds, counts = out.download()
+            _compare(cmds, counts, ec, en, ds.stateset_counts, f"{ds.scene.name} {block}x{items}")
+            assert ds.world_matrices(-1).tobytes() == world.tobytes(), f"{ds.scene.name}: world matrices"
+        b.close()
+    for ds, _, out in entries:
+        out.close(); ds.close()
