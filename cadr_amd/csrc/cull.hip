@@ -518,7 +518,7 @@ __device__ __forceinline__ void issueItemLoads(const LeafParams& Lf, uint64_t fi
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
 		const uint32_t q = uint32_t(k) * 64u + lane;  // float4 index inside the block
-		d.raw[k] = first + q / 4u < end ? loadF4(local + (4 * first + q)) : make_float4(0.f, 0.f, 0.f, 0.f);
+		d.raw[k] = first + q / 4u < end ? loadF4Stream(local + (4 * first + q)) : make_float4(0.f, 0.f, 0.f, 0.f);
 	}
 	if (live && p < Lf.parentCount) {
 		const gptr<float4> parentWorld = globalPtr<float4>(Lf.parentWorld);
