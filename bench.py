@@ -345,6 +345,9 @@ def main():
     if not args.no_hbm_probe:
         result["hbm_probe"] = hbm_probe()
         result["roofline"]["frac_of_measured_copy"] = fused_gbs / result["hbm_probe"]["copy_gbs"]
+        if "batched" in result:  # its bytes are almost all command writes
+            rb = result["batched"]["roofline"]
+            rb["frac_of_measured_write"] = rb["achieved"] / result["hbm_probe"]["write_gbs"]
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, stream, lib, N, device)
