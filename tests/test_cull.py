@@ -656,11 +656,11 @@ def test_fused_batch_random_jobs(gpu, seed):
         entries.append((ds, views, cull.CullOutputs(nv, ds.n, ds.K)))
         exp.append((worlds[-1], ec, en))
     leaves = np.concatenate(leaves)
-    for block, items in ((256, 4), (256, 8), (512, 4)):
-        print(f"fused batch random seed {seed}: {len(entries)} scenes, {block}x{items}", flush=True)
+    for block, items, flags in ((256, 4, 0), (256, 8, 0), (512, 4, 0), (256, 8, N.CULL_NO_TILE_SPHERES)):
+        print(f"fused batch random seed {seed}: {len(entries)} scenes, {block}x{items} flags {flags}", flush=True)
         for ds, _, _ in entries:
             ds.levels[-1][3].upload(np.full((ds.n, 16), np.nan, np.float32))
-        b = cull.CullBatch(cull.make_config(block, items), entries)
+        b = cull.CullBatch(cull.make_config(block, items, flags=flags), entries)
         b.set_leaves(leaves)
         b.run_fused_batch()
         synchronize()
