@@ -163,6 +163,8 @@ SIGNATURES = {
     "cadrBuildCullTiles": (c_int, [c_void_p, c_u32, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     "cadrCullWorkspaceCreate": (c_int, [c_u64, c_u32, ctypes.POINTER(c_void_p)]),
     "cadrCullWorkspaceDestroy": (c_int, [c_void_p]),
+    "cadrCullWorkspaceSetWaitLimit": (c_int, [c_void_p, c_u32]),
+    "cadrCullWorkspaceCheck": (c_int, [c_void_p]),
     "cadrCullBatch": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_u32, c_void_p, c_void_p]),
     "cadrCullView": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_void_p, c_void_p]),
     "cadrCullTileSize": (c_u32, [ctypes.POINTER(CullConfig)]),

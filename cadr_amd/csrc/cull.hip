@@ -49,8 +49,10 @@ constexpr uint32_t kViewGroup = 64;   // views whose look-backs share one wave
 constexpr uint32_t kCtlTicket = 0;    // ctl words, each on its own 128-byte line
 constexpr uint32_t kCtlDone = 32;
 constexpr uint32_t kCtlEpoch = 64;
-constexpr uint32_t kCtlWords = 96;
-constexpr uint32_t kLdsMeshes = 64;   // mesh tables up to this size are staged in LDS
+constexpr uint32_t kCtlError = 96;    // epoch of a launch whose look-back gave up, else 0
+constexpr uint32_t kCtlWords = 128;
+constexpr uint32_t kLdsMeshes = 64;
+constexpr uint64_t kDefaultWaitLimitUs = 1000000;  // 1 s: some ten thousand times a tile's run time   // mesh tables up to this size are staged in LDS
 
 constexpr uint32_t kStateAggregate = 1;
 constexpr uint32_t kStatePrefix = 2;
@@ -76,6 +78,7 @@ struct CullParams {
 	uint32_t numJobs;
 	uint32_t totalWork;
 	uint32_t flags;
+	uint64_t waitLimit;                 // steady-counter ticks a look-back may wait
 	LeafParams leaf;                    // one-view fused kernel
 	const cadrTransformLevel* leaves;   // fused batched kernel: leaf level of each job
 };
@@ -377,20 +380,51 @@ __device__ __forceinline__ bool statusReady(uint64_t s, uint32_t epoch)
 	return uint32_t(s >> 32) == epoch && (uint32_t(s) >> 30) != 0u;
 }
 
+// Look-back waits are bounded.  In a well-formed tile list every
+// predecessor holds an earlier ticket and is running or done, so a wait ends
+// within one tile's run time.  A caller-built list that names a predecessor
+// no tile publishes (a non-FIRST tile whose chainIndex - 1 is missing) would
+// wait forever; instead, a wait that lasts waitLimit ticks of the steady
+// counter records the launch's epoch in the error word and gives up, and
+// every later wait of the same launch gives up at its first unready poll.
+// The launch then ends with unspecified outputs and the workspace reports
+// CADR_ERROR_TIMEOUT (cadrCullWorkspaceCheck).  Epochs are never 0, so an
+// error word of 0 means no wait gave up.
+struct WaitBound {
+	uint32_t* ctl;
+	uint64_t limit;
+	uint32_t epoch;
+};
+
+// Called after an unready poll; start is 0 until the first such poll.
+__device__ __forceinline__ bool waitGiveUp(const WaitBound& B, uint64_t& start)
+{
+	if (__hip_atomic_load(&B.ctl[kCtlError], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == B.epoch) return true;
+	const uint64_t now = uint64_t(wall_clock64());
+	if (start == 0) {
+		start = now == 0 ? 1 : now;
+		return false;
+	}
+	if (now - start <= B.limit) return false;
+	__hip_atomic_store(&B.ctl[kCtlError], B.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return true;
+}
+
 // Exclusive prefix of chain position c, one chain per lane: walk back over the
-// predecessors adding aggregates until an inclusive prefix is found.  Each
-// predecessor holds an earlier ticket and is running or done, so every wait
-// ends; position 0 ends the walk in any case.
-__device__ uint32_t lookBackSerial(const uint64_t* chain, uint32_t c, uint32_t epoch)
+// predecessors adding aggregates until an inclusive prefix is found; position
+// 0 ends the walk in any case, and a wait bounded as above ends it early.
+__device__ uint32_t lookBackSerial(const uint64_t* chain, uint32_t c, const WaitBound& B)
 {
 	uint32_t excl = 0;
 	uint32_t p = c;
+	uint64_t start = 0;
 	while (p > 0) {
 		--p;
 		uint64_t s;
 		for (;;) {
 			s = readStatus(chain + p);
-			if (statusReady(s, epoch)) break;
+			if (statusReady(s, B.epoch)) break;
+			if (waitGiveUp(B, start)) return excl;
 			__builtin_amdgcn_s_sleep(1);
 		}
 		excl += uint32_t(s) & kValueMask;
@@ -401,20 +435,23 @@ __device__ uint32_t lookBackSerial(const uint64_t* chain, uint32_t c, uint32_t e
 
 // Exclusive prefix of chain position c with the whole wave: lane l reads
 // position p-1-l, the walk stops at the nearest inclusive prefix, and 64
-// predecessors cost one round trip.  Returns the same value in every lane.
-__device__ uint32_t lookBackWave(const uint64_t* chain, uint32_t c, uint32_t epoch, uint32_t lane)
+// predecessors cost one round trip.  Returns the same value in every lane
+// (the give-up decision is made from wave-uniform values).
+__device__ uint32_t lookBackWave(const uint64_t* chain, uint32_t c, const WaitBound& B, uint32_t lane)
 {
 	uint32_t excl = 0;
 	int64_t p = c;
+	uint64_t start = 0;
 	while (p > 0) {
 		const int64_t idx = p - 1 - int64_t(lane);
-		const uint64_t s = idx >= 0 ? readStatus(chain + idx) : packStatus(epoch, kStatePrefix, 0);
-		const bool ready = statusReady(s, epoch);
+		const uint64_t s = idx >= 0 ? readStatus(chain + idx) : packStatus(B.epoch, kStatePrefix, 0);
+		const bool ready = statusReady(s, B.epoch);
 		const uint64_t pmask = __ballot(ready && (uint32_t(s) >> 30) == kStatePrefix);
 		const uint64_t nmask = __ballot(!ready);
 		const uint32_t f = pmask ? uint32_t(__builtin_ctzll(pmask)) : 64u;
 		const uint64_t need = f >= 63u ? ~0ull : ((2ull << f) - 1ull);
 		if (nmask & need) {
+			if (uniform(uint32_t(waitGiveUp(B, start)))) return excl;
 			__builtin_amdgcn_s_sleep(1);
 			continue;
 		}
@@ -654,7 +691,7 @@ __device__ __forceinline__ void cullViewTile(const CullParams& P, const cadrCull
 			if (lane == 0) publish(chain + T.chain, packStatus(epoch, kStatePrefix, total));
 		} else {
 			if (lane == 0) publish(chain + T.chain, packStatus(epoch, kStateAggregate, total));
-			excl = lookBackWave(chain, T.chain, epoch, lane);
+			excl = lookBackWave(chain, T.chain, WaitBound{P.ctl, P.waitLimit, epoch}, lane);
 			if (lane == 0) publish(chain + T.chain, packStatus(epoch, kStatePrefix, excl + total));
 		}
 		if (lane == 0) {
@@ -990,7 +1027,7 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 					publish(chain + T.chain, packStatus(epoch, kStatePrefix, total));
 				} else {
 					publish(chain + T.chain, packStatus(epoch, kStateAggregate, total));
-					excl = lookBackSerial(chain, T.chain, epoch);
+					excl = lookBackSerial(chain, T.chain, WaitBound{P.ctl, P.waitLimit, epoch});
 					publish(chain + T.chain, packStatus(epoch, kStatePrefix, excl + total));
 				}
 				if (act) {
@@ -1136,6 +1173,8 @@ struct cadrCullWorkspace {
 	uint64_t* statusBase = nullptr;
 	uint32_t maxJobs = 0;
 	int device = 0;
+	uint64_t waitLimit = 0;      // steady-counter ticks a look-back wait may last
+	uint64_t ticksPerUs = 100;   // steady-counter rate
 };
 
 using namespace cadr;
@@ -1204,6 +1243,9 @@ CADR_API int cadrBuildCullTiles(const uint32_t* stateSetCounts, uint32_t numStat
 	uint64_t maxTiles = 0;
 	for (uint32_t k = 0; k < numStateSets; k++) {
 		const uint64_t c = stateSetCounts[k];
+		// look-back status words carry 30-bit counts
+		if (c > CADR_TILE_STATESET_MASK)
+			return setError(CADR_ERROR_INVALID_VALUE, "cadrBuildCullTiles: a StateSet holds more than 2^30-1 drawables");
 		const uint64_t nt = c == 0 ? 1 : (c + tileSize - 1) / tileSize;
 		base[k + 1] = base[k] + c;
 		firstTile[k + 1] = firstTile[k] + nt;
@@ -1255,6 +1297,14 @@ CADR_API int cadrCullWorkspaceCreate(uint64_t statusWords, uint32_t maxJobs, cad
 	// epoch 0 is never a valid epoch, so zeroed status words read as "not ready"
 	if (e == hipSuccess) e = hipMemset(w->status, 0, w->statusWords * sizeof(uint64_t));
 	if (e == hipSuccess) e = hipMemset(w->ctl, 0, kCtlWords * sizeof(uint32_t));
+	// launches start at epoch 1: 0 marks a status word never written and an
+	// error word with nothing to report
+	const uint32_t firstEpoch = 1;
+	if (e == hipSuccess) e = hipMemcpy(w->ctl + kCtlEpoch, &firstEpoch, sizeof(firstEpoch), hipMemcpyHostToDevice);
+	int khz = 0;
+	if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, w->device);
+	w->ticksPerUs = khz >= 1000 ? uint64_t(khz) / 1000u : 1u;
+	w->waitLimit = kDefaultWaitLimitUs * w->ticksPerUs;
 	if (e == hipSuccess) e = hipDeviceSynchronize();
 	if (e != hipSuccess) {
 		(void)hipGetLastError();
@@ -1264,6 +1314,33 @@ CADR_API int cadrCullWorkspaceCreate(uint64_t statusWords, uint32_t maxJobs, cad
 	}
 	*workspace = w;
 	return ok();
+}
+
+CADR_API int cadrCullWorkspaceSetWaitLimit(cadrCullWorkspace* w, uint32_t microseconds)
+{
+	CADR_REQUIRE(w, "cadrCullWorkspaceSetWaitLimit: workspace is NULL");
+	CADR_REQUIRE(microseconds > 0, "cadrCullWorkspaceSetWaitLimit: the limit must be positive");
+	w->waitLimit = uint64_t(microseconds) * w->ticksPerUs;
+	return ok();
+}
+
+CADR_API int cadrCullWorkspaceCheck(cadrCullWorkspace* w)
+{
+	CADR_REQUIRE(w, "cadrCullWorkspaceCheck: workspace is NULL");
+	int dev = 0;
+	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
+	CADR_REQUIRE(dev == w->device, "cadrCullWorkspaceCheck: workspace belongs to another device");
+	CADR_CHECK_HIP(hipDeviceSynchronize(), "cadrCullWorkspaceCheck: hipDeviceSynchronize");
+	uint32_t err = 0;
+	CADR_CHECK_HIP(hipMemcpy(&err, w->ctl + kCtlError, sizeof(err), hipMemcpyDeviceToHost),
+	               "cadrCullWorkspaceCheck: read");
+	if (err == 0) return ok();
+	const uint32_t zero = 0;
+	CADR_CHECK_HIP(hipMemcpy(w->ctl + kCtlError, &zero, sizeof(zero), hipMemcpyHostToDevice),
+	               "cadrCullWorkspaceCheck: clear");
+	return setError(CADR_ERROR_TIMEOUT,
+	                "cull: a look-back wait gave up (the tile list names a predecessor that no tile publishes); "
+	                "the outputs of launch epoch " + std::to_string(err) + " are unspecified");
 }
 
 CADR_API int cadrCullWorkspaceDestroy(cadrCullWorkspace* w)
@@ -1336,6 +1413,7 @@ int launchBatch(KernelFn fn, const KernelEntry& ke, const cadrCullConfig* config
 	p.numJobs = numJobs;
 	p.totalWork = uint32_t(totalWork);
 	p.flags = config->flags;
+	p.waitLimit = ws->waitLimit;
 	p.leaves = leaves;
 	hipLaunchKernelGGL(fn, dim3(uint32_t(grid)), dim3(ke.block), 0, s, p);
 	CADR_CHECK_HIP(hipGetLastError(), "cullKernel launch");
@@ -1393,6 +1471,7 @@ CADR_API int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, cons
 	p.numJobs = 1;
 	p.totalWork = hostJob->numTiles;
 	p.flags = config->flags;
+	p.waitLimit = ws->waitLimit;
 	hipLaunchKernelGGL(ke->view, dim3(hostJob->numTiles), dim3(256), 0, hipStream_t(stream), p);
 	CADR_CHECK_HIP(hipGetLastError(), "cullViewKernel launch");
 	return ok();
@@ -1424,6 +1503,7 @@ CADR_API int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCul
 	p.numJobs = 1;
 	p.totalWork = hostJob->numTiles;
 	p.flags = config->flags;
+	p.waitLimit = ws->waitLimit;
 	p.leaf = LeafParams{leaf->localMatrices, leaf->parentIndex, leaf->parentWorld, leaf->worldMatrices, leaf->spheres,
 	                    leaf->parentWorld ? leaf->parentCount : 0u};
 	hipLaunchKernelGGL(ke->fused, dim3(hostJob->numTiles), dim3(ke->block), 0, hipStream_t(stream), p);

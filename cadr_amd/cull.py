@@ -203,6 +203,14 @@ class Workspace:
         self.handle = w.value
         self.status_words = status_words
 
+    def set_wait_limit(self, microseconds: int) -> None:
+        N.check(N.load().cadrCullWorkspaceSetWaitLimit(self.handle, int(microseconds)))
+
+    def check(self) -> None:
+        """Waits for the device; raises CadrError (CADR_ERROR_TIMEOUT) if a
+        look-back wait of a launch on this workspace gave up."""
+        N.check(N.load().cadrCullWorkspaceCheck(self.handle))
+
     def close(self):
         if self.handle:
             N.check(N.load().cadrCullWorkspaceDestroy(self.handle))

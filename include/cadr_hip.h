@@ -335,7 +335,10 @@ typedef struct cadrCullTile {
  * most tileSize drawables, emitted round-robin over the StateSets (the q-th
  * tile of every StateSet, then the (q+1)-th, ...): consecutive tiles of one
  * StateSet are numStateSets tickets apart, which keeps look-back walks short.
- * With tiles == NULL it only counts them. */
+ * With tiles == NULL it only counts them.  A StateSet may hold at most
+ * CADR_TILE_STATESET_MASK (2^30 - 1) drawables (compaction offsets travel as
+ * 30-bit values) and a scene at most 2^32 - 1; larger counts are refused
+ * with CADR_ERROR_INVALID_VALUE. */
 int cadrBuildCullTiles(const uint32_t* stateSetCounts, uint32_t numStateSets, uint32_t tileSize,
                        cadrCullTile* tiles, uint32_t capacity, uint32_t* numTiles);
 
@@ -375,6 +378,18 @@ typedef struct cadrCullConfig {
 typedef struct cadrCullWorkspace cadrCullWorkspace;
 int cadrCullWorkspaceCreate(uint64_t statusWords, uint32_t maxJobs, cadrCullWorkspace** workspace);
 int cadrCullWorkspaceDestroy(cadrCullWorkspace* workspace);
+/* Look-back waits are bounded, so a launch ends whatever tile list it is
+ * given.  A well-formed list (cadrBuildCullTiles) only waits on tiles with
+ * earlier tickets, which are running or done; a caller-built list that names
+ * a predecessor no tile publishes would otherwise wait forever.  A wait that
+ * lasts longer than the limit (default 1 s, at least 1 us) gives up, every
+ * other wait of the same launch gives up at its next unready poll, and the
+ * launch's outputs are unspecified.  cadrCullWorkspaceCheck waits for the
+ * device and returns CADR_ERROR_TIMEOUT (CadR::Timeout) if a launch on the
+ * workspace gave up since the previous check, else CADR_SUCCESS; it clears
+ * the record. */
+int cadrCullWorkspaceSetWaitLimit(cadrCullWorkspace* workspace, uint32_t microseconds);
+int cadrCullWorkspaceCheck(cadrCullWorkspace* workspace);
 
 /* Culls numJobs jobs (job descriptors in device memory, cadrCullJob[numJobs])
  * in one persistent launch.  hostJobs is a host copy of the same

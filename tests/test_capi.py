@@ -146,3 +146,22 @@ def test_fused_batch_argument_checks_before_any_device_call():
     # well-formed leaves reach the workspace check
     assert call(ok) == N.ERROR_INVALID_VALUE
     assert "workspace is NULL" in lib.cadrGetLastErrorMessage().decode()
+
+
+def test_build_tiles_refuses_statesets_past_30_bits():
+    """Look-back status words carry 30-bit counts: a StateSet of 2^30 drawables
+    is refused on the host (counting mode, no device call)."""
+    n = ctypes.c_uint32(0)
+    lib = N.load()
+    ok = np.array([N.TILE_STATESET_MASK, 5], np.uint32)
+    assert lib.cadrBuildCullTiles(N.ptr(ok), 2, 1 << 20, None, 0, ctypes.byref(n)) == 0
+    assert n.value == 1024 + 1
+    big = np.array([5, N.TILE_STATESET_MASK + 1], np.uint32)
+    assert lib.cadrBuildCullTiles(N.ptr(big), 2, 1 << 20, None, 0, ctypes.byref(n)) == N.ERROR_INVALID_VALUE
+    assert "2^30" in lib.cadrGetLastErrorMessage().decode()
+
+
+def test_workspace_calls_check_arguments():
+    lib = N.load()
+    assert lib.cadrCullWorkspaceCheck(None) == N.ERROR_INVALID_VALUE
+    assert lib.cadrCullWorkspaceSetWaitLimit(None, 10) == N.ERROR_INVALID_VALUE
