@@ -166,6 +166,7 @@ SIGNATURES = {
     "cadrCullWorkspaceSetWaitLimit": (c_int, [c_void_p, c_u32]),
     "cadrCullWorkspaceCheck": (c_int, [c_void_p]),
     "cadrCullBatch": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_u32, c_void_p, c_void_p]),
+    "cadrCullBatchNoAlloc": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_u32, c_void_p]),
     "cadrCullView": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_void_p, c_void_p]),
     "cadrCullTileSize": (c_u32, [ctypes.POINTER(CullConfig)]),
     "cadrPropagateCullView": (c_int, [c_void_p, ctypes.POINTER(CullConfig), c_u64, c_void_p, c_void_p, c_void_p]),

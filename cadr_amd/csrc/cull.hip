@@ -40,6 +40,8 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -1509,4 +1511,103 @@ CADR_API int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCul
 	hipLaunchKernelGGL(ke->fused, dim3(hostJob->numTiles), dim3(ke->block), 0, hipStream_t(stream), p);
 	CADR_CHECK_HIP(hipGetLastError(), "fused cullViewKernel launch");
 	return ok();
+}
+
+// ------------------------------------------------ allocation-free variant
+//
+// cadrCullBatchNoAlloc: cadrCullBatch for callers that cannot allocate device
+// memory.  Its workspace (look-back status words, control words, per-job
+// offsets) is one file-scope device array, `scratch`, of fixed size; each
+// device holds its own copy until the process ends.  Host code never passes
+// the array's name to a kernel: the name is a host-side stand-in, and the
+// device address is taken once per device with hipGetSymbolAddress.
+namespace {
+
+constexpr uint64_t kScratchStatusWords = 1ull << 21;  // 16 MiB: sum of numViews x numTiles per launch
+constexpr uint32_t kScratchMaxJobs = 4096;
+// layout in 64-bit words: status | ctl (kCtlWords uint32) | firstWork (jobs+1 uint32) | statusBase (jobs uint64)
+constexpr uint64_t kScratchCtl = kScratchStatusWords;
+constexpr uint64_t kScratchFirstWork = kScratchCtl + kCtlWords / 2;
+constexpr uint64_t kScratchStatusBase = kScratchFirstWork + (uint64_t(kScratchMaxJobs) + 2) / 2;
+constexpr uint64_t kScratchWords = kScratchStatusBase + kScratchMaxJobs;
+static_assert(kCtlWords % 2 == 0, "ctl words fill whole 64-bit words");
+
+}  // namespace
+
+static __device__ __align__(256) uint64_t scratch[kScratchWords];
+
+namespace {
+
+struct ScratchDevice {
+	uint64_t* ptr = nullptr;   // device address of `scratch` on this device
+	uint64_t ticksPerUs = 100;
+};
+
+std::mutex g_scratchMutex;
+std::vector<ScratchDevice> g_scratchDevices;
+
+// Device address of this device's scratch, set up on the first call on the
+// device: the address from hipGetSymbolAddress, status and control words
+// cleared by a fill kernel (not hipMemsetAsync, so a caller may capture
+// later calls) and the epoch set to 1, complete before the call returns.
+int scratchFor(int dev, hipStream_t stream, ScratchDevice& out)
+{
+	std::lock_guard<std::mutex> lock(g_scratchMutex);
+	if (int(g_scratchDevices.size()) <= dev) g_scratchDevices.resize(size_t(dev) + 1);
+	ScratchDevice& d = g_scratchDevices[size_t(dev)];
+	if (!d.ptr) {
+		void* p = nullptr;
+		CADR_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "cadrCullBatchNoAlloc: hipGetSymbolAddress(scratch)");
+		auto* base = static_cast<uint64_t*>(p);
+		const uint64_t a = reinterpret_cast<uint64_t>(base);
+		int rc = cadrFillU32(a, 0u, kScratchStatusBase * sizeof(uint64_t), stream);
+		if (rc == CADR_SUCCESS) rc = cadrFillU32(a + kScratchCtl * 8 + kCtlEpoch * 4, 1u, 4, stream);
+		if (rc != CADR_SUCCESS) return rc;
+		CADR_CHECK_HIP(hipStreamSynchronize(stream), "cadrCullBatchNoAlloc: clearing scratch");
+		int khz = 0;
+		CADR_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
+		d.ticksPerUs = khz >= 1000 ? uint64_t(khz) / 1000u : 1u;
+		d.ptr = base;
+	}
+	out = d;
+	return ok();
+}
+
+}  // namespace
+
+CADR_API int cadrCullBatchNoAlloc(const cadrCullConfig* config, uint64_t deviceJobs, const cadrCullJob* hostJobs,
+                                  uint32_t numJobs, cadrStream stream)
+{
+	const KernelEntry* ke = findKernel(config);
+	CADR_REQUIRE(ke && ke->batch, "cadrCullBatchNoAlloc: unsupported blockSize/itemsPerThread");
+	if (numJobs == 0) return ok();
+	CADR_REQUIRE(hostJobs, "cadrCullBatchNoAlloc: hostJobs is NULL");
+	if (numJobs > kScratchMaxJobs)
+		return setError(CADR_ERROR_INVALID_VALUE, "cadrCullBatchNoAlloc: " + std::to_string(numJobs) +
+		                                              " jobs, more than the " + std::to_string(kScratchMaxJobs) +
+		                                              " its fixed scratch holds");
+	uint64_t statusNeed = 0;
+	for (uint32_t j = 0; j < numJobs; j++) statusNeed += uint64_t(hostJobs[j].numViews) * hostJobs[j].numTiles;
+	if (statusNeed > kScratchStatusWords)
+		return setError(CADR_ERROR_INVALID_VALUE,
+		                "cadrCullBatchNoAlloc: the jobs need " + std::to_string(statusNeed) +
+		                    " status words (numViews x numTiles summed over jobs), more than the " +
+		                    std::to_string(kScratchStatusWords) + " of its fixed scratch");
+	int dev = 0;
+	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
+	ScratchDevice sd;
+	const int rc = scratchFor(dev, hipStream_t(stream), sd);
+	if (rc != CADR_SUCCESS) return rc;
+	cadrCullWorkspace w;  // a view of the scratch array, nothing allocated
+	w.status = sd.ptr;
+	w.statusWords = kScratchStatusWords;
+	w.ctl = reinterpret_cast<uint32_t*>(sd.ptr + kScratchCtl);
+	w.firstWork = reinterpret_cast<uint32_t*>(sd.ptr + kScratchFirstWork);
+	w.statusBase = sd.ptr + kScratchStatusBase;
+	w.maxJobs = kScratchMaxJobs;
+	w.device = dev;
+	w.ticksPerUs = sd.ticksPerUs;
+	w.waitLimit = kDefaultWaitLimitUs * sd.ticksPerUs;
+	return launchBatch(ke->batch, *ke, config, deviceJobs, hostJobs, numJobs, &w, stream, nullptr,
+	                   "cadrCullBatchNoAlloc");
 }

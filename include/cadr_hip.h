@@ -398,6 +398,21 @@ int cadrCullWorkspaceCheck(cadrCullWorkspace* workspace);
 int cadrCullBatch(const cadrCullConfig* config, uint64_t deviceJobs, const cadrCullJob* hostJobs,
                   uint32_t numJobs, cadrCullWorkspace* workspace, cadrStream stream);
 
+/* cadrCullBatch without a workspace and without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its workspace is a fixed array in the library's device code,
+ * one per GPU, held until the process ends: at most 2^21 look-back status
+ * words (the sum over jobs of numViews x numTiles) and 4096 jobs per call;
+ * larger calls are refused with CADR_ERROR_INVALID_VALUE.  The first call on
+ * a device clears that device's array and waits for the clear.  Results are
+ * bit-identical to cadrCullBatch with the same config and jobs.  All calls on
+ * one GPU share its array: calls that overlap on one GPU (two streams, two
+ * threads) race and must be serialised by the caller.  Look-back waits are
+ * bounded as for a workspace; a give-up is not reported. */
+int cadrCullBatchNoAlloc(const cadrCullConfig* config, uint64_t deviceJobs, const cadrCullJob* hostJobs,
+                         uint32_t numJobs, cadrStream stream);
+
 /* One scene, one view: cadrCullBatch of a single job with numViews == 1. */
 int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, const cadrCullJob* hostJob,
                  cadrCullWorkspace* workspace, cadrStream stream);
