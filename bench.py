@@ -746,6 +746,7 @@ def bench_served(args, config, N, cull, scenes, device):
     runs = []
     for rate in [float(r) for r in args.served_rates.split(",") if r]:
         st = srv.run_load(views, rate=rate, duration=args.served_seconds, seed=int(rate), hot=16, hot_fraction=0.8)
+        st.pop("samples")
         st["offered_rate"] = rate
         runs.append(st)
         log(f"served128 @ {rate:.0f} req/s: {st['requestsPerSecond']:.0f} req/s, "

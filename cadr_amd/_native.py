@@ -103,13 +103,18 @@ class ServerScene(ctypes.Structure):
 
 class ServerResult(ctypes.Structure):
     _fields_ = [("requestId", c_u64), ("sceneId", c_u32), ("numStateSets", c_u32), ("commands", c_u64),
-                ("counts", c_u64), ("visible", c_u64), ("submitNs", c_u64), ("completeNs", c_u64)]
+                ("counts", c_u64), ("visible", c_u64), ("submitNs", c_u64), ("completeNs", c_u64),
+                ("status", ctypes.c_int32), ("tag", c_u32)]
+
+
+class LoadSample(ctypes.Structure):
+    _fields_ = [("result", ServerResult)]
 
 
 class LoadSpec(ctypes.Structure):
     _fields_ = [("requestsPerSecond", ctypes.c_double), ("durationSeconds", ctypes.c_double), ("seed", c_u64),
                 ("numHotScenes", c_u32), ("hotFraction", c_float), ("views", c_void_p), ("numViews", c_u32),
-                ("reserved", c_u32)]
+                ("sampleEvery", c_u32), ("samples", c_void_p), ("sampleCapacity", c_u32), ("reserved", c_u32)]
 
 
 class LoadStats(ctypes.Structure):
@@ -117,7 +122,7 @@ class LoadStats(ctypes.Structure):
                 ("requestsPerSecond", ctypes.c_double), ("drawablesPerSecond", ctypes.c_double),
                 ("p50Ms", ctypes.c_double), ("p99Ms", ctypes.c_double), ("p999Ms", ctypes.c_double),
                 ("maxMs", ctypes.c_double), ("meanMs", ctypes.c_double), ("visible", c_u64), ("launches", c_u64),
-                ("meanViewsPerLaunch", ctypes.c_double)]
+                ("meanViewsPerLaunch", ctypes.c_double), ("sampled", c_u64), ("failed", c_u64)]
 
 
 class CadrError(RuntimeError):

@@ -30,6 +30,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -114,6 +115,8 @@ struct Request {
 	uint32_t block = 0;
 	std::vector<uint32_t> hostCounts;
 	uint64_t visible = 0;
+	uint32_t tag = 0;
+	int status = CADR_SUCCESS;
 	bool done = false;
 };
 
@@ -171,13 +174,16 @@ struct cadrServer {
 	uint64_t wake = 0;          // bumped by submissions and releases
 	uint64_t wakeSeen = 0;      // value of wake at the last launch attempt
 	std::atomic<uint64_t> launches{0}, launchedViews{0};
-	int workerError = CADR_SUCCESS;
+	std::string workerMessage;  // first device failure of a launch (guarded by m)
 
 	~cadrServer();
 	void run();
 	bool retire();
 	bool launch(Slot& slot);
-	int submitAt(uint32_t scene, const cadrView* view, uint64_t arrivalNs, uint64_t* id);
+	// completes the requests of a slot (status != 0: failed, counts zero); called with m held
+	void complete(Slot& slot, int status, uint64_t t);
+	void recordFailure(const std::string& what);
+	int submitAt(uint32_t scene, const cadrView* view, uint64_t arrivalNs, uint32_t tag, uint64_t* id);
 };
 
 using namespace cadr;
@@ -202,6 +208,33 @@ cadrServer::~cadrServer()
 	for (SceneRec& sc : scenes) (void)hipFree(reinterpret_cast<void*>(sc.tiles));
 }
 
+void cadrServer::complete(Slot& s, int status, uint64_t t)
+{
+	for (size_t r = 0; r < s.reqs.size(); r++) {
+		Request* q = s.reqs[r];
+		const uint32_t K = scenes[q->scene].desc.numStateSets;
+		if (status == CADR_SUCCESS)
+			q->hostCounts.assign(s.hCounts + s.countOffset[r], s.hCounts + s.countOffset[r] + K);
+		else
+			q->hostCounts.assign(K, 0u);
+		q->visible = 0;
+		for (uint32_t c : q->hostCounts) q->visible += c;
+		q->status = status;
+		q->completeNs = t;
+		q->done = true;
+		completed.push_back(q->id);
+		inFlight--;
+	}
+	s.reqs.clear();
+	s.busy = false;
+}
+
+void cadrServer::recordFailure(const std::string& what)
+{
+	std::lock_guard<std::mutex> lock(m);
+	if (workerMessage.empty()) workerMessage = what;
+}
+
 // Moves finished launches' requests to the completion queue.
 bool cadrServer::retire()
 {
@@ -210,23 +243,10 @@ bool cadrServer::retire()
 		if (!s.busy) continue;
 		const hipError_t e = hipEventQuery(s.done);
 		if (e == hipErrorNotReady) continue;
-		if (e != hipSuccess) workerError = CADR_ERROR_DEVICE;
-		const uint64_t t = nowNs();
+		if (e != hipSuccess) recordFailure(std::string("serve: launch failed on the device: ") + hipGetErrorString(e));
 		{
 			std::lock_guard<std::mutex> lock(m);
-			for (size_t r = 0; r < s.reqs.size(); r++) {
-				Request* q = s.reqs[r];
-				const uint32_t K = scenes[q->scene].desc.numStateSets;
-				q->hostCounts.assign(s.hCounts + s.countOffset[r], s.hCounts + s.countOffset[r] + K);
-				q->visible = 0;
-				for (uint32_t c : q->hostCounts) q->visible += c;
-				q->completeNs = t;
-				q->done = true;
-				completed.push_back(q->id);
-				inFlight--;
-			}
-			s.reqs.clear();
-			s.busy = false;
+			complete(s, e == hipSuccess ? CADR_SUCCESS : CADR_ERROR_DEVICE, nowNs());
 		}
 		any = true;
 		cvDone.notify_all();
@@ -347,10 +367,20 @@ bool cadrServer::launch(Slot& s)
 		at += bytes / 4u;
 	}
 	okLaunch = okLaunch && hipEventRecord(s.done, s.stream) == hipSuccess;
-	if (!okLaunch) workerError = CADR_ERROR_DEVICE;
-	s.busy = true;
 	launches++;
 	launchedViews += nv;
+	if (!okLaunch) {
+		// nothing of this launch may be trusted: its requests complete now, failed
+		recordFailure(std::string("serve: a launch could not be issued: ") + cadrGetLastErrorMessage());
+		(void)hipGetLastError();
+		{
+			std::lock_guard<std::mutex> lock(m);
+			complete(s, CADR_ERROR_DEVICE, nowNs());
+		}
+		cvDone.notify_all();
+		return true;
+	}
+	s.busy = true;
 	return true;
 }
 
@@ -376,7 +406,7 @@ void cadrServer::run()
 	}
 }
 
-int cadrServer::submitAt(uint32_t scene, const cadrView* view, uint64_t arrivalNs, uint64_t* id)
+int cadrServer::submitAt(uint32_t scene, const cadrView* view, uint64_t arrivalNs, uint32_t tag, uint64_t* id)
 {
 	CADR_REQUIRE(view && id, "cadrServerSubmit: null argument");
 	CADR_REQUIRE(scene < scenes.size(), "cadrServerSubmit: unknown scene");
@@ -386,6 +416,7 @@ int cadrServer::submitAt(uint32_t scene, const cadrView* view, uint64_t arrivalN
 	r->scene = scene;
 	r->view = *view;
 	r->submitNs = arrivalNs;
+	r->tag = tag;
 	{
 		std::lock_guard<std::mutex> lock(m);
 		if (queue.size() >= cfg.maxQueuedRequests)
@@ -490,14 +521,13 @@ CADR_API int cadrServerStart(cadrServer* s)
 CADR_API int cadrServerSubmit(cadrServer* s, uint32_t sceneId, const cadrView* view, uint64_t* requestId)
 {
 	CADR_REQUIRE(s && s->started, "cadrServerSubmit: the server is not started");
-	return s->submitAt(sceneId, view, nowNs(), requestId);
+	return s->submitAt(sceneId, view, nowNs(), 0u, requestId);
 }
 
 CADR_API int cadrServerPoll(cadrServer* s, cadrServerResult* results, uint32_t capacity, uint32_t* count)
 {
 	CADR_REQUIRE(s && count && (capacity == 0 || results), "cadrServerPoll: null argument");
 	std::lock_guard<std::mutex> lock(s->m);
-	if (s->workerError != CADR_SUCCESS) return setError(s->workerError, "serve: a launch failed on the device");
 	uint32_t n = 0;
 	while (n < capacity && !s->completed.empty()) {
 		const uint64_t id = s->completed.front();
@@ -514,6 +544,9 @@ CADR_API int cadrServerPoll(cadrServer* s, cadrServerResult* results, uint32_t c
 		r.visible = q.visible;
 		r.submitNs = q.submitNs;
 		r.completeNs = q.completeNs;
+		r.status = q.status;
+		r.tag = q.tag;
+		if (q.status != CADR_SUCCESS) setError(q.status, s->workerMessage);
 	}
 	*count = n;
 	return CADR_SUCCESS;
@@ -579,6 +612,8 @@ CADR_API int cadrServerRunLoad(cadrServer* s, const cadrLoadSpec* spec, cadrLoad
 	CADR_REQUIRE(s && s->started && spec && stats, "cadrServerRunLoad: null argument or server not started");
 	CADR_REQUIRE(spec->views && spec->numViews > 0, "cadrServerRunLoad: no views");
 	CADR_REQUIRE(spec->requestsPerSecond > 0 && spec->durationSeconds > 0, "cadrServerRunLoad: rate and duration must be > 0");
+	CADR_REQUIRE(spec->sampleEvery == 0 || spec->sampleCapacity == 0 || spec->samples,
+	             "cadrServerRunLoad: samples is NULL");
 	const uint32_t S = uint32_t(s->scenes.size());
 	const uint32_t hot = std::min(spec->numHotScenes, S);
 	*stats = cadrLoadStats{};
@@ -613,6 +648,8 @@ CADR_API int cadrServerRunLoad(cadrServer* s, const cadrLoadSpec* spec, cadrLoad
 	const uint64_t start = nowNs() + 2000000;  // first arrival 2 ms from now
 
 	std::atomic<bool> stalled{false}, failed{false};
+	uint64_t sampled = 0, failedRequests = 0;
+	std::string failure;
 	std::thread consumer([&] {
 		std::vector<cadrServerResult> res(1024);
 		uint64_t done = 0, lastProgress = nowNs();
@@ -628,11 +665,18 @@ CADR_API int cadrServerRunLoad(cadrServer* s, const cadrLoadSpec* spec, cadrLoad
 				break;
 			}
 			for (uint32_t i = 0; i < n; i++) {
+				if (res[i].status != CADR_SUCCESS) {
+					if (failedRequests++ == 0) failure = cadrGetLastErrorMessage();
+				}
 				latMs.push_back(double(res[i].completeNs - res[i].submitNs) * 1e-6);
 				visible += res[i].visible;
 				tests += s->scenes[res[i].sceneId].desc.numDrawables;
 				lastComplete = std::max(lastComplete, res[i].completeNs);
-				cadrServerRelease(s, res[i].requestId);
+				const uint64_t k = done + i;
+				if (spec->sampleEvery && k % spec->sampleEvery == 0 && sampled < spec->sampleCapacity)
+					spec->samples[sampled++].result = res[i];  // kept: the caller releases it
+				else
+					cadrServerRelease(s, res[i].requestId);
 			}
 			done += n;
 			if (!producing && done >= submitted.load()) break;
@@ -649,15 +693,17 @@ CADR_API int cadrServerRunLoad(cadrServer* s, const cadrLoadSpec* spec, cadrLoad
 		}
 		uint64_t id = 0;
 		// latency counts from the scheduled arrival, so a late submission is not hidden
-		if (s->submitAt(a.scene, &spec->views[a.view], due, &id) == CADR_SUCCESS)
+		if (s->submitAt(a.scene, &spec->views[a.view], due, a.view, &id) == CADR_SUCCESS)
 			submitted++;
 		else
 			refused++;
 	}
 	producing = false;
 	consumer.join();
-	if (failed) return setError(CADR_ERROR_DEVICE, "cadrServerRunLoad: a launch failed on the device");
+	if (failed) return setError(CADR_ERROR_DEVICE, "cadrServerRunLoad: polling failed");
 	if (stalled) return setError(CADR_ERROR_TIMEOUT, "cadrServerRunLoad: requests stopped completing");
+	stats->sampled = sampled;
+	stats->failed = failedRequests;
 
 	stats->submitted = submitted;
 	stats->refused = refused;
@@ -679,5 +725,8 @@ CADR_API int cadrServerRunLoad(cadrServer* s, const cadrLoadSpec* spec, cadrLoad
 		stats->requestsPerSecond = double(latMs.size()) / stats->elapsedSeconds;
 		stats->drawablesPerSecond = double(tests) / stats->elapsedSeconds;
 	}
+	if (failedRequests)
+		return setError(CADR_ERROR_DEVICE, "cadrServerRunLoad: " + std::to_string(failedRequests) +
+		                                       " requests failed on the device: " + failure);
 	return CADR_SUCCESS;
 }

@@ -56,12 +56,21 @@ class Server:
     def drain(self, timeout_ms=30000):
         N.check(N.load().cadrServerDrain(self.handle, timeout_ms))
 
-    def run_load(self, views: np.ndarray, rate: float, duration: float, seed=1, hot=16, hot_fraction=0.8):
+    def run_load(self, views: np.ndarray, rate: float, duration: float, seed=1, hot=16, hot_fraction=0.8,
+                 sample_every: int = 0, sample_capacity: int = 0):
+        """Open-loop Poisson load (cadrServerRunLoad).  With sample_every > 0,
+        every sample_every-th completion (up to sample_capacity) is kept with
+        its outputs allocated and returned under "samples" (ServerResult,
+        tag = view index); the caller releases each one."""
         v = np.ascontiguousarray(views, dtype=N.VIEW_DTYPE)
-        spec = N.LoadSpec(rate, duration, seed, hot, hot_fraction, v.ctypes.data, v.shape[0], 0)
+        samples = (N.LoadSample * max(sample_capacity, 1))()
+        spec = N.LoadSpec(rate, duration, seed, hot, hot_fraction, v.ctypes.data, v.shape[0], sample_every,
+                          ctypes.cast(samples, ctypes.c_void_p), sample_capacity, 0)
         st = N.LoadStats()
         N.check(N.load().cadrServerRunLoad(self.handle, ctypes.byref(spec), ctypes.byref(st)))
-        return {f: getattr(st, f) for f, _ in N.LoadStats._fields_}
+        out = {f: getattr(st, f) for f, _ in N.LoadStats._fields_}
+        out["samples"] = [samples[i].result for i in range(st.sampled)]
+        return out
 
     def close(self):
         if self.handle:

@@ -518,6 +518,9 @@ typedef struct cadrServerResult {
 	uint64_t visible;               /* sum of the counts                         */
 	uint64_t submitNs;              /* steady-clock times of submission and of   */
 	uint64_t completeNs;            /* the counts being visible on the host      */
+	int32_t status;                 /* CADR_SUCCESS, or the error of the request's launch
+	                                 * (its outputs are then unspecified, counts 0) */
+	uint32_t tag;                   /* cadrServerRunLoad: index of the request's view; 0 otherwise */
 } cadrServerResult;
 
 int cadrServerCreate(const cadrServerConfig* config, cadrServer** server);
@@ -528,7 +531,10 @@ int cadrServerStart(cadrServer* server);
 /* Thread-safe.  CADR_ERROR_OUT_OF_RESOURCES when the queue is full or the
  * request's output can never fit the budget. */
 int cadrServerSubmit(cadrServer* server, uint32_t sceneId, const cadrView* view, uint64_t* requestId);
-/* Completed requests, oldest first; count may be 0. */
+/* Completed requests, oldest first; count may be 0.  A request whose launch
+ * failed on the device completes with a non-zero status; the message of the
+ * first such failure is kept by cadrGetLastErrorMessage of the polling
+ * thread when that request is returned. */
 int cadrServerPoll(cadrServer* server, cadrServerResult* results, uint32_t capacity, uint32_t* count);
 /* Host copy of a completed request's per-StateSet counts. */
 int cadrServerCopyCounts(cadrServer* server, uint64_t requestId, uint32_t* counts, uint32_t capacity);
@@ -540,7 +546,15 @@ int cadrServerDrain(cadrServer* server, uint32_t timeoutMs);
 /* Open-loop load: Poisson arrivals at requestsPerSecond for durationSeconds;
  * hotFraction of the requests go uniformly to scenes [0, numHotScenes), the
  * rest uniformly to the others; each request takes a view uniformly from
- * views[0..numViews).  Completed requests are released at once. */
+ * views[0..numViews).  Completed requests are released at once, except a
+ * sample: with sampleEvery > 0, every sampleEvery-th completion (in
+ * completion order, up to sampleCapacity) is kept in samples[] with its
+ * outputs still allocated, for the caller to check and then free with
+ * cadrServerRelease.  Kept outputs count against the output budget. */
+typedef struct cadrLoadSample {
+	cadrServerResult result;        /* result.tag = index of the request's view */
+} cadrLoadSample;
+
 typedef struct cadrLoadSpec {
 	double requestsPerSecond;
 	double durationSeconds;
@@ -549,6 +563,9 @@ typedef struct cadrLoadSpec {
 	float hotFraction;
 	const cadrView* views;          /* host */
 	uint32_t numViews;
+	uint32_t sampleEvery;           /* 0: keep no samples                        */
+	cadrLoadSample* samples;        /* host, sampleCapacity entries, or NULL     */
+	uint32_t sampleCapacity;
 	uint32_t reserved;
 } cadrLoadSpec;
 
@@ -561,6 +578,8 @@ typedef struct cadrLoadStats {
 	uint64_t visible;
 	uint64_t launches;
 	double meanViewsPerLaunch;
+	uint64_t sampled;               /* entries written to spec->samples          */
+	uint64_t failed;                /* completed with a non-zero status          */
 } cadrLoadStats;
 
 int cadrServerRunLoad(cadrServer* server, const cadrLoadSpec* spec, cadrLoadStats* stats);

@@ -139,9 +139,25 @@ def test_serve_load_runner(served):
     for ds in ds_list:
         srv.add_scene(ds)
     srv.start()
-    st = srv.run_load(views, rate=3000.0, duration=0.3, seed=5, hot=2, hot_fraction=0.8)
+    st = srv.run_load(views, rate=3000.0, duration=0.3, seed=5, hot=2, hot_fraction=0.8, sample_every=10,
+                      sample_capacity=40)
+    samples = st.pop("samples")
     print(st)
     assert st["refused"] == 0 and st["completed"] == st["submitted"] > 500
+    assert st["failed"] == 0 and st["sampled"] == len(samples) == 40
     assert 0 < st["p50Ms"] <= st["p99Ms"] <= st["p999Ms"] <= st["maxMs"]
     assert st["launches"] > 0 and st["meanViewsPerLaunch"] >= 1.0
+    # the kept results are the oracle's for their scene and view, whatever they were batched with
+    for r in samples:
+        s, v = r.sceneId, r.tag
+        assert r.status == 0 and v < views.shape[0]
+        cmds, counts = _result_arrays(ds_list[s], r)
+        exp_cmds, exp_counts = oracle.cull(spheres[s], ds_list[s].scene.mesh_index, ds_list[s].scene.meshes,
+                                           ds_list[s].stateset_counts, views[v:v + 1], cmd_dtype=N.CMD_DTYPE)
+        assert np.array_equal(counts, exp_counts[0]) and r.visible == counts.sum()
+        for a, b in zip(_valid(cmds, counts, ds_list[s].stateset_counts),
+                        _valid(exp_cmds[0], exp_counts[0], ds_list[s].stateset_counts)):
+            assert a.tobytes() == b.tobytes()
+        srv.release(r.requestId)
+    srv.drain()
     srv.close()
