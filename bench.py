@@ -98,9 +98,14 @@ def parse():
     p.add_argument("--no-served", action="store_true")
     p.add_argument("--no-gather", action="store_true")
     p.add_argument("--no-resident", action="store_true")
+    p.add_argument("--no-plain100k", action="store_true")
+    p.add_argument("--no-facade", action="store_true", help="skip the facade cull frames")
     p.add_argument("--no-hbm-probe", action="store_true", help="skip the measured copy / read / write bandwidth")
     p.add_argument("--resident-drawables", type=int, default=10_000_000)
-    p.add_argument("--served-rates", default="5000,10000,20000,50000", help="Poisson arrival rates (req/s)")
+    p.add_argument("--served-rates", default="5000,10000,20000,50000,100000,200000",
+                   help="Poisson arrival rates (req/s)")
+    p.add_argument("--served-overload-gib", type=float, default=1.0,
+                   help="output budget of the overload run (0: skip it)")
     p.add_argument("--served-seconds", type=float, default=1.0, help="arrival window per rate")
     p.add_argument("--served-budget-gib", type=float, default=64.0, help="output memory budget of the server")
     p.add_argument("--no-cpu", action="store_true")
@@ -327,6 +332,8 @@ def main():
         f"unfused {un_ms:.3f} ms/frame (leaf {leaf_ms:.3f} ms {leaf_gbs:.0f} GB/s, cull {cull_ms:.3f} ms); "
         f"visible {survivors1}, identical={same}")
 
+    if not args.no_plain100k:
+        result["plain100k"] = bench_plain100k(args, stream, lib, N, cull, scenes, device)
     served_config = cull.make_config(args.block, args.items)
     if not args.no_batch:
         result["batched"], served_config = bench_batched(args, ds, config, stream, lib, N, cull, scenes, device,
@@ -342,6 +349,8 @@ def main():
 
     if not args.no_resident:
         result["resident"] = bench_resident(args)
+    if not args.no_facade:
+        result["facade_cull"] = bench_facade_cull(args)
     if not args.no_hbm_probe:
         result["hbm_probe"] = hbm_probe()
         result["roofline"]["frac_of_measured_copy"] = fused_gbs / result["hbm_probe"]["copy_gbs"]
@@ -374,6 +383,32 @@ def hbm_probe():
     r = json.loads(p.stdout.strip().splitlines()[-1])
     r["what"] = "one float4 per thread over 1 GiB, 20 launches after 2 warm-up; copy counts read + write bytes"
     log(f"hbm probe: copy {r['copy_gbs']:.0f} GB/s, read {r['read_gbs']:.0f} GB/s, write {r['write_gbs']:.0f} GB/s")
+    return r
+
+
+def bench_facade_cull(args, frames=10):
+    """Cull frames through the CadR facade (cadr_amd/bin/facade_cull, C++): a
+    1000 x 100 x 100 tree whose leaf level is propagated on the GPU straight
+    into the MatrixLists of 4096 instanced Drawables, then
+    Renderer::recordDrawableProcessing + recordDrawableCulling for one view;
+    medians of HIP-event and host wall-clock frame times."""
+    import subprocess
+    exe = os.path.join(ROOT, "cadr_amd", "bin", "facade_cull")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is not built (make -C cadr_amd/csrc)")
+    roots = max(1, int(round(1000 * args.scale)))
+    p = subprocess.run([exe, str(roots), str(frames)], capture_output=True, text=True, timeout=900)
+    if p.returncode != 0:
+        raise RuntimeError(f"facade_cull failed ({p.returncode}): {p.stderr[-2000:]}")
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    r["workload"] = (f"facade cull frames: {roots} x 100 x 100 transform tree; the leaves of each (StateSet, "
+                     f"Geometry) pair are the instances of one Drawable ({r['drawables']} Drawables, "
+                     f"{r['cull_items']} cull items, 64 StateSets, 3 LODs); frame = inner level + leaf level into "
+                     f"the MatrixLists + recordDrawableProcessing + recordDrawableCulling(1 view), median of "
+                     f"{frames}")
+    r["culled_drawables_per_s"] = r["cull_items"] / (r["gpu_ms"] * 1e-3)
+    log(f"facade cull: {r['gpu_ms']:.3f} ms/frame on the GPU ({r['host_ms']:.3f} ms host), "
+        f"{r['cull_items']} items, {r['visible']} visible")
     return r
 
 
@@ -492,6 +527,13 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device, p
         f"plain {plain_ms} ms, identical={same}")
     res["frame"] = batched_frames(args, [(ds, views, out)], [leaf_desc], propagate_inner, batch, stream, lib, N, cull,
                                   f"batch{V}")
+    if not args.no_cpu:
+        ds.propagate(s)
+        stream.synchronize()
+        sph, mi = ds.download_spheres(), ds.mesh_index.download(np.uint32, n)
+        res["cpu_baseline"] = cpu_cull_rate([(sph, mi, ds.scene.meshes, ds.stateset_counts, views[:16])], 8.0,
+                                            f"batch{V} scene (10M drawables) x its first 16 views")
+        del sph, mi
     graph.close()
     batch.close(); out.close(); ws.close()
     for _, dj in plain_jobs:
@@ -702,6 +744,16 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     res["fused_frame"] = batched_frames(args, entries, [lvs[-1] for lvs in levels], propagate_inner, batch, stream,
                                         lib, N, cull, "uneven64", unfused=lambda: (propagate_all(), batch.run(s)))
     res["fused_frame"]["culled_drawables_per_s"] = tests / (res["fused_frame"]["fused_ms_per_frame"] * 1e-3)
+    if not args.no_cpu:
+        # the first 8 scenes' inputs (generated on the device) read back
+        samples = []
+        for ds, views in zip(dss[:8], views_l[:8]):
+            lv = [(lb.download(np.float32, 16 * c).reshape(c, 16), None if pb is None else pb.download(np.uint32, c))
+                  for c, lb, pb, _ in ds.levels]
+            samples.append((lv, ds.mesh_index.download(np.uint32, ds.n), ds.scene.meshes, ds.stateset_counts, views))
+        res["cpu_baseline"] = cpu_frame_rate(samples, 10.0, f"uneven64 scenes 0-7 ({sum(x[1].shape[0] for x in samples)} "
+                                                            f"drawables, {sum(x[4].shape[0] for x in samples)} views)")
+        del samples
     frame.close(); cull_only.close()
     for _, dj in plain:
         dj.close()
@@ -752,10 +804,36 @@ def bench_served(args, config, N, cull, scenes, device):
         log(f"served128 @ {rate:.0f} req/s: {st['requestsPerSecond']:.0f} req/s, "
             f"{st['drawablesPerSecond'] / 1e9:.1f} G tests/s, p50 {st['p50Ms']:.3f} ms, p99 {st['p99Ms']:.3f} ms, "
             f"p999 {st['p999Ms']:.3f} ms, {st['meanViewsPerLaunch']:.1f} views/launch")
+    # saturation: the highest offered rate whose p99 stays under 2 ms
+    under = [r["offered_rate"] for r in runs if r["p99Ms"] < 2.0 and r["refused"] == 0]
     srv.close()
+    # overload: a small output budget binds, so requests queue for memory
+    # and the bounded queue refuses the excess
+    overload = None
+    if args.served_overload_gib > 0:
+        small = Server(block=config.blockSize, items=config.itemsPerThread, flags=config.flags, max_batch_views=256,
+                       inflight=4, budget_bytes=int(args.served_overload_gib * (1 << 30)), max_queued=4096)
+        for ds in dss:
+            small.add_scene(ds)
+        small.start()
+        overload = small.run_load(views, rate=100_000.0, duration=args.served_seconds, seed=777, hot=16,
+                                  hot_fraction=0.8)
+        overload.pop("samples")
+        overload.update({"offered_rate": 100_000.0, "output_budget_gib": args.served_overload_gib,
+                         "max_queued_requests": 4096})
+        log(f"served128 overload (budget {args.served_overload_gib} GiB, queue 4096) @ 100k req/s: "
+            f"{overload['requestsPerSecond']:.0f} req/s done, {overload['refused']} refused, "
+            f"p50 {overload['p50Ms']:.3f} ms, p99 {overload['p99Ms']:.3f} ms")
+        small.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_cull_rate([(ds.download_spheres(), ds.mesh_index.download(np.uint32, ds.n), ds.scene.meshes,
+                              ds.stateset_counts, views[i:i + 1]) for i, ds in enumerate(dss[:16])], 5.0,
+                            "served128: the 16 hottest scenes, one request (view) each")
     for ds in dss:
         ds.close()
-    return {"workload": "served128: 128 flat scenes of round(exp(U[ln 1e4, ln 2e7])) drawables (64 StateSets, 3 LODs), "
+    return {"max_rate_p99_under_2ms": max(under) if under else None, "overload": overload, "cpu_baseline": cpu,
+            "workload": "served128: 128 flat scenes of round(exp(U[ln 1e4, ln 2e7])) drawables (64 StateSets, 3 LODs), "
                         "Poisson arrivals, 80% of requests to the 16 hottest scenes, views as batch256",
             "drawables_resident": int(sum(sizes)), "output_budget_gib": args.served_budget_gib,
             "max_batch_views": 256, "inflight_batches": 4,
@@ -928,17 +1006,157 @@ class Graph:
         self.exec = self.graph = None
 
 
-def cpu_baseline(scale: float = 0.25, budget_s: float = 12.0):
+def cpu_threads() -> int:
+    """Threads of the CPU baselines: the host's cores, at most 16 (one GPU's
+    share of a shared host)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "unknown CPU")
+    except OSError:
+        return "unknown CPU"
+
+
+def cpu_cull_rate(scenes_views, budget_s: float, what: str):
+    """The oracle's cull (OpenMP over (view, StateSet) pairs) on the host cores
+    over a bounded sample: scenes_views = [(spheres, mesh_index, meshes,
+    stateset_counts, views)], all culled once per repetition into reused output
+    arrays; median repetition; (drawable, view) tests per second."""
+    import oracle
+    from cadr_amd import _native as N
+    threads = cpu_threads()
+    outs = [(np.zeros((v.shape[0], sp.shape[0]), N.CMD_DTYPE), np.zeros((v.shape[0], len(c)), np.uint32))
+            for sp, _, _, c, v in scenes_views]
+    tests = sum(sp.shape[0] * v.shape[0] for sp, _, _, _, v in scenes_views)
+    times = []
+    start = time.perf_counter()
+    while len(times) < 3 or (time.perf_counter() - start < budget_s and len(times) < 200):
+        t0 = time.perf_counter()
+        for (sp, mi, me, c, v), o in zip(scenes_views, outs):
+            oracle.cull(sp, mi, me, c, v, cmd_dtype=N.CMD_DTYPE, nthreads=threads, out=o)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    log(f"cpu baseline ({what}): {tests / t / 1e6:.0f} M tests/s ({len(times)} repetitions, {threads} threads)")
+    return {"value": tests / t, "unit": "culled drawables/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: {tests} (drawable, view) tests per repetition, median of {len(times)}; "
+                      f"oracle/cadr_oracle.c cull, gcc -O2, {threads} OpenMP threads of {os.cpu_count()} "
+                      f"({cpu_model()})"}
+
+
+def cpu_frame_rate(samples, budget_s: float, what: str):
+    """Whole frames on the host cores: samples = [(levels [(local, parent)],
+    mesh_index, meshes, stateset_counts, views)], each propagated level by
+    level (the last level writing world spheres) and culled for its views once
+    per repetition; median repetition; (drawable, view) tests per second."""
+    import oracle
+    from cadr_amd import _native as N
+    threads = cpu_threads()
+    oracle.set_threads(threads)
+    outs = [(np.zeros((v.shape[0], mi.shape[0]), N.CMD_DTYPE), np.zeros((v.shape[0], len(c)), np.uint32))
+            for _, mi, _, c, v in samples]
+    tests = sum(mi.shape[0] * v.shape[0] for _, mi, _, _, v in samples)
+    times = []
+    start = time.perf_counter()
+    try:
+        while len(times) < 3 or (time.perf_counter() - start < budget_s and len(times) < 200):
+            t0 = time.perf_counter()
+            for (levels, mi, me, c, v), o in zip(samples, outs):
+                prev = None
+                for d, (local, parent) in enumerate(levels):
+                    last = d == len(levels) - 1
+                    prev, sph = oracle.propagate_level(local, parent, prev, mi if last else None, me if last else None)
+                oracle.cull(sph, mi, me, c, v, cmd_dtype=N.CMD_DTYPE, nthreads=threads, out=o)
+            times.append(time.perf_counter() - t0)
+    finally:
+        oracle.set_threads(1)
+    t = float(np.median(times))
+    log(f"cpu baseline ({what}): {tests / t / 1e6:.0f} M tests/s ({len(times)} repetitions, {threads} threads)")
+    return {"value": tests / t, "unit": "culled drawables/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: whole frames (propagation + cull), {tests} (drawable, view) tests per repetition, "
+                      f"median of {len(times)}; oracle/cadr_oracle.c, gcc -O2, {threads} OpenMP threads of "
+                      f"{os.cpu_count()} ({cpu_model()})"}
+
+
+def bench_plain100k(args, stream, lib, N, cull, scenes, device):
+    """plain100k: the workload's 100k-drawable flat scene (one StateSet, the
+    default camera) culled for one view by the single-view kernel; launch-
+    bound at this size.  Per-launch time from HIP events around single
+    launches; back-to-back throughput from one event pair around 200 launches
+    and from a HIP graph of 100 launches; the oracle on the same scene and
+    view beside it."""
+    from cadr_amd.device import Event
+    from cadr_amd.tune import autotune, describe
+    import oracle
+    s = stream.handle
+    scene = scenes.plain_scene(100_000)
+    ds = cull.DeviceScene(scene)
+    ds.propagate(s)
+    views = cull.make_views([scenes.DEFAULT_CAMERA])
+    out = cull.CullOutputs(1, ds.n, ds.K)
+    config, table = cull.make_config(256, 4), None
+    if not args.no_tune:
+        config, table = autotune([(ds, views, out)], single_view=True, reps=20, stream=s)
+    b = cull.CullBatch(config, [(ds, views, out)])
+    for _ in range(20):
+        b.run_view(s)
+    stream.synchronize()
+    pairs = [(Event(), Event()) for _ in range(200)]
+    for e0, e1 in pairs:
+        e0.record(s)
+        b.run_view(s)
+        e1.record(s)
+    stream.synchronize()
+    single = sorted(e0.elapsed_ms(e1) for e0, e1 in pairs)
+    e0, e1 = Event(), Event()
+    e0.record(s)
+    for _ in range(200):
+        b.run_view(s)
+    e1.record(s)
+    stream.synchronize()
+    back_to_back = e0.elapsed_ms(e1) / 200
+    graph_ms = time_graph(lambda: [b.run_view(s) for _ in range(100)], stream, 5) / 100
+    counts = out.download()[1]
+    survivors = int(counts.sum())
+    _, spheres = oracle.propagate_scene(scene)
+    exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
+                                       cmd_dtype=N.CMD_DTYPE)
+    cmds = out.download()[0]
+    identical = bool(np.array_equal(counts, exp_counts) and
+                     cmds[0, :survivors].tobytes() == exp_cmds[0, :survivors].tobytes())
+    alg = ds.n * 20 + survivors * 20 + ds.K * 4
+    med = single[len(single) // 2]
+    res = {"workload": "plain100k: 100k drawables (64 meshes, translation U[-500,500]^3, scale log-uniform [0.1,2]), "
+                       "1 StateSet, camera at the origin looking -z, vfov 60 deg, 16:9; one single-view launch",
+           "drawables": ds.n, "visible": survivors, "identical_to_oracle": identical,
+           "launch_us_median": 1e3 * med, "launch_us_min": 1e3 * single[0],
+           "back_to_back_us": 1e3 * back_to_back, "graph_us": 1e3 * graph_ms,
+           "culled_drawables_per_s": ds.n / (med * 1e-3),
+           "culled_drawables_per_s_graph": ds.n / (graph_ms * 1e-3),
+           "algorithmic_bytes": alg, "hbm_frac_graph": alg / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "bound": "launch / latency: 2.2 MB per launch stays in the caches",
+           "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread,
+                      "autotune": None if table is None else {tune_key(t): t["ms"] for t in table}},
+           "cpu_baseline": cpu_cull_rate([(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views)],
+                                         3.0, "plain100k, 1 view")}
+    log(f"plain100k: {1e3 * med:.1f} us per launch (median), {1e3 * graph_ms:.1f} us per launch in a graph, "
+        f"{survivors} visible, identical to the oracle: {identical}")
+    b.close(); out.close(); ds.close()
+    return res
+
+
+def cpu_baseline(scale: float = 1.0, budget_s: float = 15.0):
     """The oracle (C restatement, gcc -O2, OpenMP over nodes and over
-    (view, StateSet) pairs) on the host cores, on a bounded sample of the
-    workload: the full10m structure with a quarter of the roots (2.5M
-    drawables), whole frames (propagation + cull of one view) repeated until
-    about budget_s seconds; median frame time.  Threads: the host's cores, at
-    most 16 (one GPU's share of a shared host)."""
+    (view, StateSet) pairs) on the host cores on the headline workload itself
+    (full10m, 10M drawables): whole frames (propagation + cull of one view)
+    repeated until about budget_s seconds; median frame time.  Threads: the
+    host's cores, at most 16 (one GPU's share of a shared host)."""
     import oracle
     from cadr_amd import _native as N
     from cadr_amd import cull, scenes
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     oracle.set_threads(threads)
     scene = scenes.full10m_scene(scale)
     views = cull.make_views([scenes.DEFAULT_CAMERA])
@@ -954,15 +1172,10 @@ def cpu_baseline(scale: float = 0.25, budget_s: float = 12.0):
     finally:
         oracle.set_threads(1)
     t = float(np.median(times))
-    model = "unknown CPU"
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), model)
-    except OSError:
-        pass
+    model = cpu_model()
     log(f"cpu baseline: {t:.3f} s per {scene.num_drawables}-drawable frame ({len(times)} frames, {threads} threads)")
     return {"value": scene.num_drawables / t, "unit": "drawables/s", "cores": threads, "kind": "port",
-            "sample": f"full10m structure at {scale:g} of the roots ({scene.num_drawables} drawables), whole frames "
+            "sample": f"full10m at {scale:g} of the roots ({scene.num_drawables} drawables), whole frames "
                       f"(propagation + cull of 1 view), median of {len(times)}; oracle/cadr_oracle.c, gcc -O2, "
                       f"{threads} OpenMP threads of {os.cpu_count()} ({model})"}
 

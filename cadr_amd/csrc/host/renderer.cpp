@@ -65,6 +65,7 @@ void Renderer::finalize()
 		*e = nullptr;
 	}
 	_uploadPending = _listFreePending = false;
+	releaseCull();
 	if (_delta) (void)hipHostFree(_delta);
 	if (_deltaDevice) (void)hipFree(reinterpret_cast<void*>(_deltaDevice));
 	_delta = nullptr;
@@ -202,6 +203,8 @@ void Renderer::recordDrawableProcessing(void* stream, size_t numDrawables)
 		throw LogicError("CadR::Renderer::recordDrawableProcessing(): numDrawables differs from the prepared scene");
 
 	_lastUploadBytes = 0;
+	_processedFrame = _frameNumber;
+	_processedCount = numDrawables;
 	if (numDrawables == 0) {
 		// two timestamps stand in for the skipped work (Renderer.cpp:600-620)
 		recordEvent(1, stream);
@@ -344,6 +347,302 @@ void Renderer::recordSceneRendering(void*, StateSet& root, std::vector<DrawRange
 {
 	if (&root != _preparedRoot) throw LogicError("CadR::Renderer::recordSceneRendering(): root was not prepared");
 	ranges = _pendingRanges;
+	if (_culledFrame != _frameNumber) {
+		for (DrawRange& r : ranges) {  // no cull frame: the items are the drawables
+			r.itemBase = r.base;
+			r.itemCount = r.count;
+		}
+	}
+}
+
+// ------------------------------------------------------------ cull frame
+
+uint32_t Renderer::meshIndexOf(const Geometry& g, uint32_t primitiveSetOffset)
+{
+	const auto key = std::make_pair(&g, primitiveSetOffset);
+	const auto it = _meshIndex.find(key);
+	if (it != _meshIndex.end()) return it->second;
+	const uint32_t m = uint32_t(_meshKeys.size());
+	_meshKeys.push_back(key);
+	_meshIndex.emplace(key, m);
+	return m;
+}
+
+void Renderer::forgetGeometry(const Geometry* g) noexcept
+{
+	auto it = _meshIndex.lower_bound(std::make_pair(g, 0u));
+	while (it != _meshIndex.end() && it->first.first == g) {
+		_meshKeys[it->second].first = nullptr;
+		it = _meshIndex.erase(it);
+	}
+}
+
+void Renderer::setCullConfig(const cadrCullConfig& singleView, const cadrCullConfig& batched)
+{
+	if (cadrCullTileSize(&singleView) == 0 || cadrCullTileSize(&batched) == 0)
+		throw LogicError("CadR::Renderer::setCullConfig(): unsupported blockSize / itemsPerThread");
+	_cullSingle = singleView;
+	_cullBatched = batched;
+}
+
+void Renderer::growCull(uint64_t& buffer, size_t& capacity, size_t bytes, const char* what)
+{
+	if (bytes <= capacity && buffer) return;
+	(void)hipFree(reinterpret_cast<void*>(buffer));
+	buffer = 0;
+	capacity = 0;
+	const size_t cap = std::max<size_t>(256, bytes + bytes / 5);
+	void* p = nullptr;
+	hipCheck(hipMalloc(&p, cap), what);
+	buffer = reinterpret_cast<uint64_t>(p);
+	capacity = cap;
+}
+
+void Renderer::releaseCull() noexcept
+{
+	for (uint64_t* b : {&_cull.spheres, &_cull.meshIndex, &_cull.itemBase, &_cull.itemSource, &_cull.rangeBase,
+	                    &_cull.tiles, &_cull.views, &_cull.job, &_cull.commands, &_cull.counts, &_cull.pointers,
+	                    &_cull.sources, &_cull.meshes})
+		if (*b) (void)hipFree(reinterpret_cast<void*>(*b));
+	_cull = CullBuffers{};
+	if (_cullCopied) (void)hipEventDestroy(hipEvent_t(_cullCopied));
+	_cullCopied = nullptr;
+	if (_cullPinned) (void)hipHostFree(_cullPinned);
+	_cullPinned = nullptr;
+	_cullPinnedCap = 0;
+	_cullCopyPending = false;
+	if (_cullWs) cadrCullWorkspaceDestroy(_cullWs);
+	_cullWs = nullptr;
+	_cullWsWords = 0;
+	_cullLayoutEpoch = 0;
+	_cullRanges.clear();
+	_cullTileSize = 0;
+}
+
+// Item layout of the frame's ranges: one cull item per matrix of each
+// drawable's MatrixList, in slot order.  Rebuilt only when a drawable or a
+// MatrixList size changed (scene epoch) or the ranges moved; with exactly one
+// matrix per drawable the items are the drawable slots themselves.
+void Renderer::updateCullLayout(void* stream)
+{
+	(void)stream;
+	bool same = _cullLayoutEpoch == _sceneEpoch && _cullRanges.size() == _pendingRanges.size();
+	for (size_t k = 0; same && k < _pendingRanges.size(); k++)
+		same = _cullRanges[k].stateSet == _pendingRanges[k].stateSet && _cullRanges[k].base == _pendingRanges[k].base &&
+		       _cullRanges[k].count == _pendingRanges[k].count;
+	if (same) {
+		_pendingRanges = _cullRanges;
+		return;
+	}
+	bool identity = true;
+	for (const DrawRange& r : _pendingRanges)
+		for (size_t j = 0; identity && j < r.count; j++) identity = r.stateSet->getDrawable(j).matrixList()->numMatrices() == 1;
+	size_t items = 0;
+	std::vector<uint32_t> itemBase, itemDrawable, rangeBase(std::max<size_t>(_pendingRanges.size(), 1), 0u);
+	if (!identity) itemBase.resize(_processedCount);
+	for (size_t k = 0; k < _pendingRanges.size(); k++) {
+		DrawRange& r = _pendingRanges[k];
+		r.itemBase = items;
+		if (identity) {
+			items += r.count;
+		} else {
+			for (size_t j = 0; j < r.count; j++) {
+				itemBase[r.base + j] = uint32_t(items);
+				const size_t m = r.stateSet->getDrawable(j).matrixList()->numMatrices();
+				itemDrawable.insert(itemDrawable.end(), m, uint32_t(r.base + j));
+				items += m;
+			}
+		}
+		r.itemCount = items - r.itemBase;
+		rangeBase[k] = uint32_t(r.itemBase);
+		if (items >= 0xffffffffull) throw OutOfResources("CadR::Renderer::recordDrawableCulling(): more than 2^32 - 1 cull items");
+	}
+	if (!identity) {
+		growCull(_cull.itemBase, _cull.itemBaseCap, itemBase.size() * 4, "hipMalloc(cull item bases)");
+		hipCheck(hipMemcpy(reinterpret_cast<void*>(_cull.itemBase), itemBase.data(), itemBase.size() * 4,
+		                   hipMemcpyHostToDevice),
+		         "hipMemcpy(cull item bases)");
+		growCull(_cull.itemSource, _cull.itemSourceCap, std::max<size_t>(items, 1) * 4, "hipMalloc(cull item drawables)");
+		if (items)
+			hipCheck(hipMemcpy(reinterpret_cast<void*>(_cull.itemSource), itemDrawable.data(), items * 4,
+			                   hipMemcpyHostToDevice),
+			         "hipMemcpy(cull item drawables)");
+	}
+	growCull(_cull.rangeBase, _cull.rangeBaseCap, rangeBase.size() * 4, "hipMalloc(cull range bases)");
+	hipCheck(hipMemcpy(reinterpret_cast<void*>(_cull.rangeBase), rangeBase.data(), rangeBase.size() * 4,
+	                   hipMemcpyHostToDevice),
+	         "hipMemcpy(cull range bases)");
+	growCull(_cull.spheres, _cull.spheresCap, items * 16, "hipMalloc(cull spheres)");
+	growCull(_cull.meshIndex, _cull.meshIndexCap, items * 4, "hipMalloc(cull mesh indices)");
+	_cullIdentity = identity;
+	_cullItems = items;
+	_cullRanges = _pendingRanges;
+	_cullLayoutEpoch = _sceneEpoch;
+	_cullTileSize = 0;  // tiles follow the item counts
+}
+
+void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t numViews)
+{
+	const char* fn = "CadR::Renderer::recordDrawableCulling";
+	if (_device < 0) throw LogicError(std::string(fn) + "(): the renderer has no device");
+	if (_processedFrame != _frameNumber)
+		throw LogicError(std::string(fn) + "(): recordDrawableProcessing() of this frame must come first");
+	if (numViews == 0 || !views) throw LogicError(std::string(fn) + "(): no views");
+	if (numViews > 0xffffffffull) throw LogicError(std::string(fn) + "(): too many views");
+	const hipStream_t s = hipStream_t(stream);
+	const cadrCullConfig& cfg = numViews == 1 ? _cullSingle : _cullBatched;
+	const uint32_t tile = cadrCullTileSize(&cfg);
+	updateCullLayout(stream);
+	const size_t N = _processedCount, items = _cullItems, K = _cullRanges.size(), V = numViews;
+	_cullViews = V;
+	_culledFrame = _frameNumber;
+	if (K == 0) return;
+	// the host copies below are read by the last frame's async copies until they ran
+	if (!_cullCopied) {
+		hipEvent_t ev = nullptr;
+		hipCheck(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreateWithFlags");
+		_cullCopied = ev;
+	}
+	if (_cullCopyPending) hipCheck(hipEventSynchronize(hipEvent_t(_cullCopied)), "hipEventSynchronize");
+	_cullCopyPending = false;
+	// pinned staging: job, views, mesh sources (pageable copies would wait for the stream)
+	const size_t M = _meshKeys.size();
+	const size_t viewsAt = 128, sourcesAt = viewsAt + (V * sizeof(cadrView) + 127) / 128 * 128;
+	const size_t pinnedBytes = sourcesAt + std::max<size_t>(M, 1) * sizeof(cadrMeshSource);
+	if (_cullPinnedCap < pinnedBytes) {
+		if (_cullPinned) (void)hipHostFree(_cullPinned);
+		_cullPinned = nullptr;
+		_cullPinnedCap = 0;
+		void* p = nullptr;
+		hipCheck(hipHostMalloc(&p, pinnedBytes + pinnedBytes / 4, hipHostMallocDefault), "hipHostMalloc(cull staging)");
+		_cullPinned = static_cast<uint8_t*>(p);
+		_cullPinnedCap = pinnedBytes + pinnedBytes / 4;
+	}
+	auto* src = reinterpret_cast<cadrMeshSource*>(_cullPinned + sourcesAt);
+	auto* viewsHost = reinterpret_cast<cadrView*>(_cullPinned + viewsAt);
+	cadrCullJob& job = *reinterpret_cast<cadrCullJob*>(_cullPinned);
+	if (items * V > (size_t(1) << 40) / 20)
+		throw OutOfResources(std::string(fn) + "(): the per-view command buffers would exceed 1 TiB");
+
+	// mesh table: each (Geometry, PrimitiveSet offset) with its current sphere and LODs
+	for (size_t m = 0; m < M; m++) {
+		cadrMeshSource& o = src[m];
+		o = cadrMeshSource{};
+		const Geometry* g = _meshKeys[m].first;
+		const uint32_t off = _meshKeys[m].second;
+		if (!g) continue;  // gone: never visible
+		const size_t psBytes = g->primitiveSetDataAllocation().size();
+		const size_t fit = psBytes >= off + 8u ? (psBytes - off) / 8u : 0u;  // LOD records inside the allocation
+		o.primitiveSetHandle = g->primitiveSetDataAllocation().handle();
+		o.primitiveSetOffset = off;
+		o.lodCount = uint32_t(std::min<size_t>(g->numLods(), fit));
+		const BoundingSphere& b = g->boundingSphere();
+		o.sphere[0] = b.center[0]; o.sphere[1] = b.center[1]; o.sphere[2] = b.center[2]; o.sphere[3] = b.radius;
+	}
+	growCull(_cull.sources, _cull.sourcesCap, std::max<size_t>(M, 1) * sizeof(cadrMeshSource), "hipMalloc(mesh sources)");
+	growCull(_cull.meshes, _cull.meshesCap, std::max<size_t>(M, 1) * sizeof(cadrMesh), "hipMalloc(mesh table)");
+	hipCheck(hipMemcpyAsync(reinterpret_cast<void*>(_cull.sources), src, M * sizeof(cadrMeshSource),
+	                        hipMemcpyHostToDevice, s),
+	         "hipMemcpyAsync(mesh sources)");
+	const uint64_t table = _dataStorage->handleTableDeviceAddress();
+	const uint32_t level = _dataStorage->handleLevel();
+	throwOnError(cadrBuildMeshTable(table, level, _cull.sources, uint32_t(M), _cull.meshes, stream), fn);
+
+	// cull items from the resident drawable list
+	if (items > 0) {
+		cadrCullItemsDesc d{};
+		d.handleTable = table;
+		d.drawableList = _drawableBuffer;
+		d.itemBase = _cullIdentity ? 0 : _cull.itemBase;
+		d.itemDrawable = _cullIdentity ? 0 : _cull.itemSource;
+		d.meshes = _cull.meshes;
+		d.spheres = _cull.spheres;
+		d.meshIndex = _cull.meshIndex;
+		d.numDrawables = N;
+		d.numItems = items;
+		d.handleLevel = level;
+		d.numMeshes = uint32_t(M);
+		throwOnError(cadrBuildCullItems(&d, stream), fn);
+	}
+
+	// outputs, views, tiles and the job
+	growCull(_cull.commands, _cull.commandsCap, std::max<size_t>(V * items * 20, 4), "hipMalloc(cull commands)");
+	growCull(_cull.counts, _cull.countsCap, V * K * 4, "hipMalloc(cull counts)");
+	if (_cullPointersOn)
+		growCull(_cull.pointers, _cull.pointersCap, std::max<size_t>(V * items * 32, 16), "hipMalloc(cull pointers)");
+	growCull(_cull.views, _cull.viewsCap, V * sizeof(cadrView), "hipMalloc(cull views)");
+	std::memcpy(viewsHost, views, V * sizeof(cadrView));
+	hipCheck(hipMemcpyAsync(reinterpret_cast<void*>(_cull.views), viewsHost, V * sizeof(cadrView),
+	                        hipMemcpyHostToDevice, s),
+	         "hipMemcpyAsync(cull views)");
+	if (_cullTileSize != tile) {
+		std::vector<uint32_t> counts(K);
+		for (size_t k = 0; k < K; k++) counts[k] = uint32_t(_cullRanges[k].itemCount);
+		uint32_t nt = 0;
+		throwOnError(cadrBuildCullTiles(counts.data(), uint32_t(K), tile, nullptr, 0, &nt), fn);
+		std::vector<cadrCullTile> tiles(nt);
+		throwOnError(cadrBuildCullTiles(counts.data(), uint32_t(K), tile, tiles.data(), nt, &nt), fn);
+		growCull(_cull.tiles, _cull.tilesCap, nt * sizeof(cadrCullTile), "hipMalloc(cull tiles)");
+		hipCheck(hipMemcpy(reinterpret_cast<void*>(_cull.tiles), tiles.data(), nt * sizeof(cadrCullTile),
+		                   hipMemcpyHostToDevice),
+		         "hipMemcpy(cull tiles)");
+		_cullNumTiles = nt;
+		_cullTileSize = tile;
+	}
+	const uint64_t words = uint64_t(V) * _cullNumTiles;
+	if (!_cullWs || _cullWsWords < words) {
+		if (_cullWs) {
+			hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+			cadrCullWorkspaceDestroy(_cullWs);
+			_cullWs = nullptr;
+		}
+		throwOnError(cadrCullWorkspaceCreate(words + words / 4, 1, &_cullWs), fn);
+		_cullWsWords = words + words / 4;
+	}
+	job = cadrCullJob{};
+	job.spheres = _cull.spheres;
+	job.meshIndex = _cull.meshIndex;
+	job.meshes = _cull.meshes;
+	job.tiles = _cull.tiles;
+	job.views = _cull.views;
+	job.commands = _cull.commands;
+	job.counts = _cull.counts;
+	job.numDrawables = uint32_t(items);
+	job.numTiles = _cullNumTiles;
+	job.numStateSets = uint32_t(K);
+	job.numViews = uint32_t(V);
+	job.commandStride = uint32_t(items);
+	job.numMeshes = uint32_t(M);
+	growCull(_cull.job, _cull.jobCap, sizeof(cadrCullJob), "hipMalloc(cull job)");
+	hipCheck(hipMemcpyAsync(reinterpret_cast<void*>(_cull.job), &job, sizeof(job), hipMemcpyHostToDevice, s),
+	         "hipMemcpyAsync(cull job)");
+	hipCheck(hipEventRecord(hipEvent_t(_cullCopied), s), "hipEventRecord");
+	_cullCopyPending = true;
+	if (V == 1)
+		throwOnError(cadrCullView(&cfg, _cull.job, &job, _cullWs, stream), fn);
+	else
+		throwOnError(cadrCullBatch(&cfg, _cull.job, &job, 1, _cullWs, stream), fn);
+
+	// the reference's draw contract: instance index and DrawablePointers per command
+	throwOnError(cadrFinishCullCommands(_cull.commands, _cull.counts, _cull.rangeBase, uint32_t(V), uint32_t(K),
+	                                    uint32_t(items), _cullIdentity ? 0 : _cull.itemBase,
+	                                    _cullIdentity ? 0 : _cull.itemSource, items, _pointersBuffer,
+	                                    N, _cullPointersOn ? _cull.pointers : 0, stream),
+	             fn);
+	for (size_t k = 0; k < K; k++) {
+		_pendingRanges[k].itemBase = _cullRanges[k].itemBase;
+		_pendingRanges[k].itemCount = _cullRanges[k].itemCount;
+	}
+}
+
+void Renderer::downloadCullCounts(std::vector<uint32_t>& counts) const
+{
+	counts.assign(_cullViews * _cullRanges.size(), 0u);
+	if (counts.empty()) return;
+	hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+	hipCheck(hipMemcpy(counts.data(), reinterpret_cast<const void*>(_cull.counts), counts.size() * 4,
+	                   hipMemcpyDeviceToHost),
+	         "hipMemcpy(cull counts)");
 }
 
 void Renderer::endRecording(void* stream) { recordEvent(3, stream); }

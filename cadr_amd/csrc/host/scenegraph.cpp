@@ -11,19 +11,32 @@
 namespace CadR {
 
 Geometry::Geometry(Renderer& r)
-    : _vertices(r.dataStorage()), _indices(r.dataStorage()), _primitiveSets(r.dataStorage())
+    : _renderer(&r), _vertices(r.dataStorage()), _indices(r.dataStorage()), _primitiveSets(r.dataStorage())
 {
+}
+
+// the renderer's mesh entries of this Geometry stop existing (never visible)
+Geometry::~Geometry() noexcept { _renderer->forgetGeometry(this); }
+
+void Geometry::setNumLods(uint32_t n)
+{
+	if (n < 1 || n > 4) throw LogicError("CadR::Geometry::setNumLods(): the LOD count must be 1 to 4");
+	_numLods = n;
 }
 
 // ------------------------------------------------------------ MatrixList
 
-MatrixList::MatrixList(Renderer& r) : _matrixList(r.dataStorage()) {}
+MatrixList::MatrixList(Renderer& r) : _renderer(&r), _matrixList(r.dataStorage()) {}
 
 // Header {numMatrices, capacity = numMatrices, 0 x 7} then the matrices
 // (MatrixList.h:54-59: initZeroField writes both counts and zeroes the rest).
 float* MatrixList::editNewContent(size_t numMatrices)
 {
 	auto* p = static_cast<uint8_t*>(_matrixList.alloc(64 + 64 * numMatrices));
+	if (numMatrices != _numMatrices) {
+		_numMatrices = numMatrices;
+		_renderer->sceneChanged();  // the cull items of its drawables change
+	}
 	uint32_t hdr[16] = {};
 	hdr[0] = uint32_t(numMatrices);
 	hdr[1] = uint32_t(numMatrices);
@@ -39,10 +52,14 @@ void MatrixList::setMatrices(const float* matrices, size_t numMatrices)
 
 // -------------------------------------------------------------- Drawable
 
+// The padding word carries the renderer's mesh index of (g, off) for the cull
+// frame; the gather does not read it.
 static DrawableGpuData gpuDataOf(Geometry& g, uint32_t off, MatrixList& ml, DataAllocation* dd)
 {
-	return DrawableGpuData(g.vertexDataAllocation().handle(), g.indexDataAllocation().handle(), ml.handle(),
-	                       dd ? dd->handle() : 0, g.primitiveSetDataAllocation().handle(), off);
+	DrawableGpuData d(g.vertexDataAllocation().handle(), g.indexDataAllocation().handle(), ml.handle(),
+	                  dd ? dd->handle() : 0, g.primitiveSetDataAllocation().handle(), off);
+	d.padding = g.renderer().meshIndexOf(g, off);
+	return d;
 }
 
 Drawable::Drawable(Geometry& g, uint32_t off, MatrixList& ml, StateSet& ss) : _matrixList(&ml)
@@ -172,6 +189,7 @@ StateSet::~StateSet() noexcept { removeAllDrawables(); }
 
 void StateSet::markDirty(uint32_t index) noexcept
 {
+	_renderer->sceneChanged();
 	if (_allDirty) return;
 	if (_dirty.size() >= std::max<size_t>(64, _drawableDataList.size() / 4)) {
 		_allDirty = true;
@@ -210,6 +228,7 @@ void StateSet::removeDrawableInternal(Drawable& d) noexcept
 	}
 	_drawableDataList.pop_back();
 	_drawablePtrList.pop_back();
+	_renderer->sceneChanged();
 }
 
 void StateSet::appendDrawable(Drawable& d, const DrawableGpuData& gpuData)
@@ -230,6 +249,7 @@ void StateSet::removeAllDrawables() noexcept
 	for (Drawable* d : _drawablePtrList) d->_indexIntoStateSet = ~0u;
 	_drawableDataList.clear();
 	_drawablePtrList.clear();
+	_renderer->sceneChanged();
 }
 
 // StateSet::prepareRecording (StateSet.cpp:180-195): drawables of this

@@ -97,46 +97,159 @@ DataStorage::DataStorage(Renderer& r, MemoryProvider& provider)
 
 DataStorage::~DataStorage() noexcept
 {
-	for (Block& b : _blocks)
+	// the handle table (a member, destroyed after this body) frees its tables
+	// into blocks that are gone by then
+	for (Block& b : _blocks) {
 		if (b.base) _provider->release(b.base);
+		b.base = 0;
+	}
 }
 
 // Alignment ladder of the reference's allocator: 64 B for allocations of 64 B
 // and more, 16 B otherwise (CircularAllocationMemory.h:565-568).
 static size_t alignmentFor(size_t numBytes) { return numBytes >= 64 ? 64 : 16; }
 
-// Block choice of DataStorage::allocInternal (DataStorage.cpp:31-101): the
-// first allocation block is tried, then the second; when neither has room, a
-// new large block becomes the second and the old second the first.  The
-// first block is 64 KiB (or larger for a large first allocation), the second
-// 2 MiB, later ones 32 MiB (Renderer.h:104-106).  A block whose allocations
-// are all freed is reused from its start, so an emptied storage places new
-// allocations from the start of its first block again.
-uint32_t DataStorage::blockFor(size_t numBytes, size_t align)
+// The smallest hole that holds numBytes at an aligned start, else the tail.
+// Holes of exactly numBytes or a little more may not fit once aligned, so a
+// few candidates are tried before one of numBytes + align - 1 is taken, which
+// always fits.
+bool DataStorage::fit(const Block& b, size_t numBytes, size_t align, size_t& offset)
 {
-	auto fits = [&](uint32_t i) {
-		const Block& b = _blocks[i];
-		const size_t at = (b.top + align - 1) / align * align;
-		return at + numBytes <= b.size;
-	};
-	auto create = [&](size_t size) {
-		Block b;
-		b.base = _provider->allocate(size);
-		b.size = size;
-		_blocks.push_back(b);
-		return uint32_t(_blocks.size() - 1);
-	};
+	if (!b.freeBySize.empty()) {
+		auto it = b.freeBySize.lower_bound({numBytes, 0});
+		for (int tries = 0; it != b.freeBySize.end() && tries < 8; ++it, ++tries) {
+			const size_t at = (it->second + align - 1) / align * align;
+			if (at + numBytes <= it->second + it->first) {
+				offset = at;
+				return true;
+			}
+		}
+		it = b.freeBySize.lower_bound({numBytes + align - 1, 0});
+		if (it != b.freeBySize.end()) {
+			offset = (it->second + align - 1) / align * align;
+			return true;
+		}
+	}
+	const size_t at = (b.tail + align - 1) / align * align;
+	if (at + numBytes > b.size) return false;
+	offset = at;
+	return true;
+}
+
+namespace {
+
+// A hole [off, off + len) in both indexes, reusing the nodes of `byOffset`
+// and `bySize` when given (node handles of a hole just taken out).
+template <class M, class S>
+void putHole(M& byOffset, S& bySize, size_t off, size_t len, typename M::node_type&& o = {},
+             typename S::node_type&& z = {})
+{
+	if (o) {
+		o.key() = off;
+		o.mapped() = len;
+		byOffset.insert(std::move(o));
+	} else {
+		byOffset.emplace(off, len);
+	}
+	if (z) {
+		z.value() = {len, off};
+		bySize.insert(std::move(z));
+	} else {
+		bySize.emplace(len, off);
+	}
+}
+
+}  // namespace
+
+// Removes [at, at + numBytes) from the free space holding it.  Space left
+// after it stays free; the alignment padding before it (under 64 bytes)
+// goes with the allocation and is returned with it.
+size_t DataStorage::takeRange(Block& b, size_t at, size_t numBytes)
+{
+	if (at >= b.tail) {
+		const size_t pad = at - b.tail;
+		b.tail = at + numBytes;
+		return pad;
+	}
+	auto r = std::prev(b.freeByOffset.upper_bound(at));
+	const size_t start = r->first, len = r->second, end = at + numBytes, rend = start + len;
+	auto o = b.freeByOffset.extract(r);
+	auto z = b.freeBySize.extract({len, start});
+	if (rend > end) putHole(b.freeByOffset, b.freeBySize, end, rend - end, std::move(o), std::move(z));
+	return at - start;
+}
+
+// Gives [at, at + numBytes) back: to the tail when it ends there, else as a
+// hole merged with free neighbours.
+void DataStorage::returnRange(Block& b, size_t at, size_t numBytes)
+{
+	size_t start = at, end = at + numBytes;
+	auto next = b.freeByOffset.lower_bound(at);
+	decltype(b.freeByOffset.extract(next)) o;
+	decltype(b.freeBySize.extract({0, 0})) z;
+	if (next != b.freeByOffset.end() && next->first == end) {
+		end += next->second;
+		z = b.freeBySize.extract({next->second, next->first});
+		auto after = std::next(next);
+		o = b.freeByOffset.extract(next);
+		next = after;
+	}
+	if (next != b.freeByOffset.begin()) {
+		auto prev = std::prev(next);
+		if (prev->first + prev->second == start) {
+			start = prev->first;
+			auto z2 = b.freeBySize.extract({prev->second, prev->first});
+			auto o2 = b.freeByOffset.extract(prev);
+			if (!o) o = std::move(o2);
+			if (!z) z = std::move(z2);
+		}
+	}
+	if (end == b.tail) {
+		b.tail = start;  // the space joins the tail; a hole right before it was merged above
+		return;
+	}
+	putHole(b.freeByOffset, b.freeBySize, start, end - start, std::move(o), std::move(z));
+}
+
+uint32_t DataStorage::newBlock(size_t size)
+{
+	Block b;
+	b.base = _provider->allocate(size);
+	b.size = size;
+	_liveBlocks++;
+	for (uint32_t i = 0; i < _blocks.size(); i++)
+		if (_blocks[i].base == 0) {
+			_blocks[i] = std::move(b);
+			return i;
+		}
+	_blocks.push_back(std::move(b));
+	return uint32_t(_blocks.size() - 1);
+}
+
+// Block choice of DataStorage::allocInternal (DataStorage.cpp:31-101): the
+// first allocation block is tried, then the second; the first block is
+// 64 KiB (or larger for a large first allocation), the second 2 MiB.  When
+// neither has room, the other blocks still holding allocations are tried in
+// turn (the reference never returns to them), and only then a new large
+// block (32 MiB or the allocation's size) becomes the second and the old
+// second the first.
+uint32_t DataStorage::blockFor(size_t numBytes, size_t align, size_t& offset)
+{
 	if (_firstBlock == ~0u)
-		_firstBlock = create(numBytes < smallMemorySize    ? smallMemorySize
-		                     : numBytes < mediumMemorySize ? mediumMemorySize
-		                                                   : std::max(numBytes, largeMemorySize));
-	if (fits(_firstBlock)) return _firstBlock;
+		_firstBlock = newBlock(numBytes < smallMemorySize    ? smallMemorySize
+		                       : numBytes < mediumMemorySize ? mediumMemorySize
+		                                                     : std::max(numBytes, largeMemorySize));
+	if (fit(_blocks[_firstBlock], numBytes, align, offset)) return _firstBlock;
 	if (_secondBlock == ~0u)
-		_secondBlock = create(numBytes < mediumMemorySize ? mediumMemorySize : std::max(numBytes, largeMemorySize));
-	if (fits(_secondBlock)) return _secondBlock;
-	const uint32_t b = create(std::max(numBytes, largeMemorySize));
+		_secondBlock = newBlock(numBytes < mediumMemorySize ? mediumMemorySize : std::max(numBytes, largeMemorySize));
+	if (fit(_blocks[_secondBlock], numBytes, align, offset)) return _secondBlock;
+	for (uint32_t i = 0; i < _blocks.size(); i++)
+		if (_blocks[i].base && i != _firstBlock && i != _secondBlock && fit(_blocks[i], numBytes, align, offset))
+			return i;
+	const uint32_t b = newBlock(std::max(numBytes, largeMemorySize));
 	_firstBlock = _secondBlock;
 	_secondBlock = b;
+	offset = 0;
 	return b;
 }
 
@@ -144,11 +257,9 @@ DataAllocationRecord* DataStorage::alloc(size_t numBytes)
 {
 	if (numBytes == 0) return &_zeroRecord;  // null-object pattern (DataStorage.cpp:125-128)
 	const size_t align = alignmentFor(numBytes);
-	const uint32_t bi = blockFor(numBytes, align);
+	size_t at = 0;
+	const uint32_t bi = blockFor(numBytes, align, at);
 	Block& b = _blocks[bi];
-	const size_t at = (b.top + align - 1) / align * align;
-	b.top = at + numBytes;
-	b.live++;
 	DataAllocationRecord* r;
 	if (!_freeRecords.empty()) {
 		r = _freeRecords.back();
@@ -157,10 +268,13 @@ DataAllocationRecord* DataStorage::alloc(size_t numBytes)
 		_recordPool.push_back(std::make_unique<DataAllocationRecord>());
 		r = _recordPool.back().get();
 	}
+	const size_t pad = takeRange(b, at, numBytes);
+	b.live++;
 	*r = DataAllocationRecord{};
 	r->deviceAddress = b.base + at;
 	r->size = numBytes;
 	r->block = bi;
+	r->padding = uint32_t(pad);
 	r->stagingFrameNumber = _renderer->frameNumber();
 	_live++;
 	return r;
@@ -188,11 +302,18 @@ void DataStorage::free(DataAllocationRecord* r) noexcept
 		last->pendingIndex = r->pendingIndex;
 		_pending.pop_back();
 	}
-	Block& b = _blocks[r->block];
-	if (--b.live == 0) b.top = 0;  // an empty block is reused from its start
+	const uint32_t bi = r->block;
+	Block& b = _blocks[bi];
+	if (b.base) returnRange(b, size_t(r->deviceAddress - b.base) - r->padding, r->size + r->padding);
 	_live--;
 	*r = DataAllocationRecord{};
 	_freeRecords.push_back(r);
+	if (--b.live == 0 && bi != _firstBlock && bi != _secondBlock && b.base) {
+		// a block outside the allocation order that holds nothing goes back
+		_provider->release(b.base);
+		b = Block{};
+		_liveBlocks--;
+	}
 }
 
 uint8_t* DataStorage::stage(DataAllocationRecord* r)

@@ -23,7 +23,10 @@
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <set>
+#include <utility>
 #include <vector>
 
 namespace CadR {
@@ -47,6 +50,7 @@ struct DataAllocationRecord {
 	uint64_t deviceAddress = 0;
 	size_t size = 0;
 	uint32_t block = ~0u;             // owning memory block
+	uint32_t padding = 0;             // alignment bytes before deviceAddress that belong to it
 	size_t stagingFrameNumber = size_t(-2);
 	uint8_t* stagingData = nullptr;   // host bytes to upload, valid until executeCopyOperations()
 	size_t pendingIndex = 0;          // position in the storage's upload list while stagingData is set
@@ -118,7 +122,7 @@ public:
 	uint64_t handleTableDeviceAddress() const { return _handleTable.rootTableDeviceAddress(); }
 	const HandleTable& handleTable() const { return _handleTable; }
 
-	size_t numBlocks() const { return _blocks.size(); }
+	size_t numBlocks() const { return _liveBlocks; }
 	size_t liveAllocations() const { return _live; }
 
 	static constexpr size_t smallMemorySize = 64 << 10;   // Renderer.h:104-106
@@ -126,14 +130,32 @@ public:
 	static constexpr size_t largeMemorySize = 32 << 20;
 
 private:
+	// Free space of a block: the space past its last allocation (`tail`) and
+	// coalesced holes before it, indexed by offset and by size.  An
+	// allocation takes the smallest hole it fits in (at its aligned start),
+	// else the tail, so an empty block is filled front to back with the alignment
+	// ladder of CircularAllocationMemory.h:565-568, as the reference places
+	// consecutive allocations, and space freed anywhere in the block is used
+	// again (the reference's circular buffer reuses only what lies behind its
+	// oldest live allocation, CircularAllocationMemory.h:522-560).
 	struct Block {
-		uint64_t base = 0;
+		uint64_t base = 0;   // 0: released
 		size_t size = 0;
-		size_t top = 0;      // bump offset
-		size_t live = 0;     // live allocations; the block is reset when it reaches 0
+		size_t live = 0;
+		size_t tail = 0;     // [tail, size) is free; the ranges below hold the holes before it
+		std::map<size_t, size_t> freeByOffset;           // offset -> size
+		std::set<std::pair<size_t, size_t>> freeBySize;  // (size, offset)
 	};
-	uint32_t blockFor(size_t numBytes, size_t align);
+	// aligned offset of a new allocation in block b, or false
+	static bool fit(const Block& b, size_t numBytes, size_t align, size_t& offset);
+	// takes [at, at + numBytes); returns the alignment padding before `at`
+	// that goes with it (free space too small to matter)
+	static size_t takeRange(Block& b, size_t at, size_t numBytes);
+	static void returnRange(Block& b, size_t at, size_t numBytes);
+	uint32_t blockFor(size_t numBytes, size_t align, size_t& offset);
+	uint32_t newBlock(size_t size);
 	uint32_t _firstBlock = ~0u, _secondBlock = ~0u;  // allocation order of DataStorage.cpp:31-101
+	size_t _liveBlocks = 0;
 
 	Renderer* _renderer;
 	MemoryProvider* _provider;

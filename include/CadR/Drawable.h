@@ -11,6 +11,11 @@
 //     element into the hole (StateSet.cpp:29-47); child StateSets are visited
 //     depth first, and a StateSet reachable through several parents is
 //     visited once per path (StateSet.cpp:180-268).
+// For the cull frame (Renderer::recordDrawableCulling, an extension: the
+// reference draws every drawable) a Geometry also carries its local bounding
+// sphere and LOD count (LOD k is the PrimitiveSet record at the drawable's
+// offset + 8k), and the record's padding word holds the renderer's mesh
+// index of the drawable's (Geometry, PrimitiveSet offset) pair.
 #pragma once
 
 #include "Export.h"
@@ -48,11 +53,29 @@ struct PrimitiveSet {
 	uint32_t startIndex;
 };
 
+// Local bounding sphere (src/CadR/BoundingSphere.h:18-43: glm::vec3 centre and
+// a float radius; the empty sphere has radius -inf).
+struct BoundingSphere {
+	float center[3];
+	float radius;
+	static constexpr BoundingSphere empty() { return BoundingSphere{{0.f, 0.f, 0.f}, -__builtin_huge_valf()}; }
+	bool isEmpty() const { return radius == -__builtin_huge_valf(); }
+};
+
 class CADR_EXPORT Geometry {
 public:
 	explicit Geometry(Renderer& r);
+	~Geometry() noexcept;
 	Geometry(const Geometry&) = delete;
 	Geometry& operator=(const Geometry&) = delete;
+
+	// culling: local bounding sphere (empty by default: never visible) and the
+	// number of LODs, PrimitiveSet records at offset, offset + 8, ... (1..4)
+	void setBoundingSphere(const BoundingSphere& s) { _boundingSphere = s; }
+	const BoundingSphere& boundingSphere() const { return _boundingSphere; }
+	void setNumLods(uint32_t n);
+	uint32_t numLods() const { return _numLods; }
+	Renderer& renderer() const { return *_renderer; }
 
 	DataAllocation& vertexDataAllocation() { return _vertices; }
 	DataAllocation& indexDataAllocation() { return _indices; }
@@ -70,7 +93,10 @@ public:
 	void freeData() { _vertices.free(); _indices.free(); _primitiveSets.free(); }
 
 private:
+	Renderer* _renderer;
 	DataAllocation _vertices, _indices, _primitiveSets;
+	BoundingSphere _boundingSphere = BoundingSphere::empty();
+	uint32_t _numLods = 1;
 };
 
 class CADR_EXPORT MatrixList {
@@ -81,9 +107,15 @@ public:
 	float* editNewContent(size_t numMatrices);
 	uint64_t handle() const { return _matrixList.handle(); }
 	uint64_t deviceAddress() const { return _matrixList.deviceAddress(); }
+	// the matrices follow the 64-byte header: a device array of numMatrices()
+	// column-major mat4 that a GPU transform pass may write directly
+	uint64_t matricesDeviceAddress() const { return _matrixList.deviceAddress() + 64; }
+	size_t numMatrices() const { return _numMatrices; }
 
 private:
+	Renderer* _renderer;
 	DataAllocation _matrixList;
+	size_t _numMatrices = 0;
 };
 
 class CADR_EXPORT Drawable {
@@ -108,11 +140,13 @@ public:
 
 private:
 	void createInternal(Geometry& g, uint32_t off, MatrixList& ml, DataAllocation* dd, StateSet& ss);
+	const MatrixList* matrixList() const { return _matrixList; }
 	StateSet* _stateSet = nullptr;
 	MatrixList* _matrixList = nullptr;
 	DataAllocation* _drawableData = nullptr;
 	uint32_t _indexIntoStateSet = ~0u;
 	friend class StateSet;
+	friend class Renderer;
 };
 
 // Parent-child relation of StateSets (ParentChildList.h): one edge per
@@ -181,11 +215,17 @@ using ChildList = StateSetList<true>;
 using ParentList = StateSetList<false>;
 
 // One StateSet's share of the indirect buffers, as StateSet::recordToCommandBuffer
-// issues it: commands [base, base + count) (StateSet.cpp:233-265).
+// issues it: commands [base, base + count) (StateSet.cpp:233-265).  After a
+// cull frame, the StateSet's cull items (one per matrix of each drawable's
+// MatrixList) are [itemBase, itemBase + itemCount) of each view's culled
+// command range, of which the first `count` of the view's count buffer are
+// written; without culling itemBase = base and itemCount = count.
 struct DrawRange {
 	StateSet* stateSet;
 	size_t base;
 	size_t count;
+	size_t itemBase = 0;
+	size_t itemCount = 0;
 };
 
 class CADR_EXPORT StateSet {

@@ -30,6 +30,30 @@
 // Renderer.cpp:635-644).  drawableBufferAddress() can change from frame to
 // frame in resident mode.
 //
+// Cull frame (an extension: the reference draws every drawable).
+// recordDrawableCulling(stream, views, n), after recordDrawableProcessing,
+// culls the frame's drawables for n views on the GPU, selects each one's LOD
+// and compacts the survivors per StateSet and view:
+//   - one cull item per matrix of each drawable's MatrixList (an instance),
+//     in the draw-range order; its world sphere is the Geometry's bounding
+//     sphere under that matrix (BoundingSphere.h:70-87), computed on the
+//     device from the resident drawable list through the handle table;
+//   - LOD k of a drawable is the PrimitiveSet record at its offset + 8k
+//     (the Geometry's numLods() of them), chosen by projected radius;
+//   - for view w, StateSet range k's visible items are compacted to the front
+//     of [itemBase_k, itemBase_k + itemCount_k) of the view's
+//     VkDrawIndexedIndirectCommand array (cullCommandBufferAddress() +
+//     w * cullCommandStride() * 20), their number at
+//     cullCountBufferAddress() + (w * numRanges + k) * 4: the operands of
+//     vkCmdDrawIndexedIndirectCount;
+//   - each command draws one instance: firstInstance is the matrix index in
+//     the drawable's MatrixList (matrices[gl_InstanceIndex] in the reference's
+//     shaders), and with setCullDrawablePointers(true) the drawable's
+//     DrawablePointers record (from the gather) sits at the command's slot of
+//     cullPointersBufferAddress(), for DrawablePointers[gl_DrawID]
+//     (examples/RenderingPerformance/shader.vert:99-123).
+// recordSceneRendering then reports itemBase / itemCount of every range.
+//
 // FrameInfo carries the reference's fields (FrameInfo.h:15-32); GPU
 // timestamps are HIP events converted to nanoseconds from the frame's first
 // GPU event.
@@ -38,11 +62,14 @@
 #include "Export.h"
 
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "DataStorage.h"
 #include "Drawable.h"
+#include "../cadr_hip.h"
 
 namespace CadR {
 
@@ -110,6 +137,29 @@ public:
 	bool residentDrawables() const { return _resident; }
 	size_t lastDrawableUploadBytes() const { return _lastUploadBytes; }
 
+	// cull frame (see above)
+	void recordDrawableCulling(void* stream, const cadrView* views, size_t numViews);
+	void setCullConfig(const cadrCullConfig& singleView, const cadrCullConfig& batched);
+	void setCullDrawablePointers(bool on) { _cullPointersOn = on; }
+	uint64_t cullCommandBufferAddress() const { return _cull.commands; }
+	uint64_t cullCountBufferAddress() const { return _cull.counts; }
+	uint64_t cullPointersBufferAddress() const { return _cullPointersOn ? _cull.pointers : 0; }
+	uint64_t cullSphereBufferAddress() const { return _cull.spheres; }
+	uint64_t cullMeshTableAddress() const { return _cull.meshes; }
+	size_t cullCommandStride() const { return _cullItems; }
+	size_t numCullItems() const { return _cullItems; }
+	size_t numCullViews() const { return _cullViews; }
+	size_t numCullRanges() const { return _cullRanges.size(); }
+	size_t numMeshes() const { return _meshKeys.size(); }
+	// counts of the last cull frame, [view][range] (waits for the device)
+	void downloadCullCounts(std::vector<uint32_t>& counts) const;
+	// bookkeeping of the scene graph: the mesh index of a (Geometry,
+	// PrimitiveSet offset) pair, a Geometry that goes away, and any change of
+	// drawables or MatrixList sizes (the cull items are rebuilt)
+	uint32_t meshIndexOf(const Geometry& g, uint32_t primitiveSetOffset);
+	void forgetGeometry(const Geometry* g) noexcept;
+	void sceneChanged() noexcept { ++_sceneEpoch; }
+
 	// frame timing (Renderer.cpp:1025-1066)
 	void setCollectFrameInfo(bool on) { _collectFrameInfo = on; }
 	bool collectFrameInfo() const { return _collectFrameInfo; }
@@ -124,6 +174,9 @@ private:
 	void uploadFullList(void* stream, size_t numDrawables);
 	void uploadResidentList(void* stream, const std::vector<DrawRange>& ranges);
 	void reserveDelta(size_t count);
+	void updateCullLayout(void* stream);
+	void growCull(uint64_t& buffer, size_t& capacity, size_t bytes, const char* what);
+	void releaseCull() noexcept;
 
 	static Renderer* _defaultRenderer;
 	int _device = -1;
@@ -153,6 +206,37 @@ private:
 	void* _listFree = nullptr;               // the last gather has read the drawable list
 	bool _listFreePending = false;
 	size_t _lastUploadBytes = 0;
+
+	// cull frame
+	size_t _processedFrame = size_t(-1);     // frame of the last recordDrawableProcessing
+	size_t _culledFrame = size_t(-1);        // frame of the last recordDrawableCulling
+	// pinned host copies of the job, views and mesh sources, read by async
+	// copies: rewritten only once _cullCopied (recorded after those copies)
+	// completed
+	uint8_t* _cullPinned = nullptr;
+	size_t _cullPinnedCap = 0;
+	void* _cullCopied = nullptr;
+	bool _cullCopyPending = false;
+	size_t _processedCount = 0;
+	uint64_t _sceneEpoch = 1;
+	uint64_t _cullLayoutEpoch = 0;          // _sceneEpoch the item layout was built for
+	std::vector<DrawRange> _cullRanges;      // ranges the item layout was built for
+	bool _cullIdentity = true;               // one item per drawable
+	size_t _cullItems = 0, _cullViews = 0;
+	uint32_t _cullTileSize = 0;
+	uint32_t _cullNumTiles = 0;
+	bool _cullPointersOn = false;
+	cadrCullConfig _cullSingle{256, 16, 0, 0}, _cullBatched{256, 8, 0, 0};
+	struct CullBuffers {
+		uint64_t spheres = 0, meshIndex = 0, itemBase = 0, itemSource = 0, rangeBase = 0, tiles = 0;
+		uint64_t views = 0, job = 0, commands = 0, counts = 0, pointers = 0, sources = 0, meshes = 0;
+		size_t spheresCap = 0, meshIndexCap = 0, itemBaseCap = 0, itemSourceCap = 0, rangeBaseCap = 0, tilesCap = 0;
+		size_t viewsCap = 0, jobCap = 0, commandsCap = 0, countsCap = 0, pointersCap = 0, sourcesCap = 0, meshesCap = 0;
+	} _cull;
+	cadrCullWorkspace* _cullWs = nullptr;
+	uint64_t _cullWsWords = 0;
+	std::map<std::pair<const Geometry*, uint32_t>, uint32_t> _meshIndex;
+	std::vector<std::pair<const Geometry*, uint32_t>> _meshKeys;   // geometry null once it is gone
 };
 
 }  // namespace CadR

@@ -479,6 +479,78 @@ int cadrCullAutotune(const cadrCullTuneJob* jobs, uint32_t numJobs, uint32_t fla
                      cadrCullConfig* best, cadrStream stream);
 
 /* ------------------------------------------------------------------------ */
+/* Cull inputs from the reference's own records, and the reference's draw
+ * contract for the compacted commands (the CadR facade's cull frame,
+ * Renderer::recordDrawableCulling).  The reference binds per drawable a
+ * Geometry's PrimitiveSet allocation at a byte offset and a MatrixList
+ * (DrawableGpuData, src/CadR/Drawable.h:32-43), and its consumer reads
+ * DrawablePointers[gl_DrawID] and matrices[gl_InstanceIndex]
+ * (examples/RenderingPerformance/shader.vert:99-123).  No reference
+ * counterpart: the reference draws every drawable.
+ *
+ * A mesh of the cull's table is one (Geometry, PrimitiveSet offset) pair with
+ * its local bounding sphere and LOD count; LOD k is the PrimitiveSet record
+ * {count, first} at lookup(primitiveSetHandle) + primitiveSetOffset + 8k,
+ * read as processDrawables.comp:97 reads LOD 0. */
+typedef struct cadrMeshSource {
+	uint64_t primitiveSetHandle;
+	uint32_t primitiveSetOffset;
+	uint32_t lodCount;             /* 1..4 (larger is 4); 0: the mesh no longer exists */
+	float    sphere[4];            /* local bounding sphere {centre xyz, radius}       */
+} cadrMeshSource;                  /* 32 B */
+
+/* meshes[m] = the cadrMesh of sources[m]: its sphere and lodCount,
+ * vertexOffset 0, indexCount / firstIndex of each LOD < lodCount (zero
+ * beyond).  A source with lodCount 0, or whose PrimitiveSet handle resolves
+ * to 0, gets the empty sphere (radius -inf) and one zero LOD: its drawables
+ * are never visible.  sources and meshes are device arrays (16-byte aligned). */
+int cadrBuildMeshTable(uint64_t handleTable, uint32_t handleLevel, uint64_t sources, uint32_t count,
+                       uint64_t meshes, cadrStream stream);
+
+/* Cull items of a device drawable list.  Item `it` stands for matrix j of
+ * drawable i's MatrixList (one instance): with itemBase, i = itemDrawable[it]
+ * and j = it - itemBase[i] (the items of a drawable are consecutive); without
+ * it, item i is matrix 0 of drawable i.
+ *   spheres[it]   = world sphere of meshes[d.padding] under matrix j
+ *                   (BoundingSphere.h:70-87, as cadrComputeWorldSpheres), or
+ *                   the empty sphere {0, 0, 0, -inf} when j >= numMatrices of
+ *                   the list, the MatrixList handle resolves to 0,
+ *                   d.padding >= numMeshes or i >= numDrawables;
+ *   meshIndex[it] = d.padding (the mesh index, kept by the facade in the
+ *                   record's padding word), ~0 for i >= numDrawables. */
+typedef struct cadrCullItemsDesc {
+	uint64_t handleTable;          /* root table (DataStorage::handleTableDeviceAddress) */
+	uint64_t drawableList;         /* cadrDrawableGpuData[numDrawables]                  */
+	uint64_t itemBase;             /* uint32[numDrawables], or 0: one item per drawable  */
+	uint64_t itemDrawable;         /* uint32[numItems] (with itemBase), or 0             */
+	uint64_t meshes;               /* cadrMesh[numMeshes]                                */
+	uint64_t spheres;              /* float4[numItems], written                          */
+	uint64_t meshIndex;            /* uint32[numItems], written                          */
+	uint64_t numDrawables;
+	uint64_t numItems;
+	uint32_t handleLevel;
+	uint32_t numMeshes;
+} cadrCullItemsDesc;
+
+int cadrBuildCullItems(const cadrCullItemsDesc* desc, cadrStream stream);
+
+/* The reference's draw contract for compacted commands of cull items.  For
+ * view w and StateSet k (base_k = stateSetBase[k], n = counts[w*K + k]) and
+ * j < n, with c = commands[w*commandStride + base_k + j], item = c.firstInstance,
+ * i = itemDrawable[item] and m = item - itemBase[i] (i = item, m = 0 without
+ * itemBase):
+ *   c.firstInstance = m      (gl_InstanceIndex = m reads the item's matrix)
+ *   pointersOut[w*commandStride + base_k + j] = pointersIn[i]
+ *                            (the drawable's DrawablePointers, from
+ *                            cadrProcessDrawables; only when pointersOut != 0;
+ *                            zeros for i >= numDrawables)
+ * A command whose item index is at or past numItems is left as it is. */
+int cadrFinishCullCommands(uint64_t commands, uint64_t counts, uint64_t stateSetBase, uint32_t numViews,
+                           uint32_t numStateSets, uint32_t commandStride, uint64_t itemBase, uint64_t itemDrawable,
+                           uint64_t numItems, uint64_t pointersIn, uint64_t numDrawables, uint64_t pointersOut,
+                           cadrStream stream);
+
+/* ------------------------------------------------------------------------ */
 /* Serving: scenes resident in HBM, cull requests (scene, view) admitted by
  * output memory and batch slots, and batched continuously — every launch
  * takes the requests that are waiting when a slot frees, groups them by

@@ -198,15 +198,20 @@ def make_view(cam, view_dtype):
     return out
 
 
-def cull(spheres, mesh_index, meshes, stateset_counts, views, stride=None, cmd_dtype=None, nthreads=1):
+def cull(spheres, mesh_index, meshes, stateset_counts, views, stride=None, cmd_dtype=None, nthreads=1, out=None):
+    """out: optional (commands (V, stride), counts (V, K)) arrays to write into."""
     spheres = np.ascontiguousarray(spheres, np.float32)
     n = spheres.shape[0]
     stride = n if stride is None else stride
     counts_in = np.ascontiguousarray(stateset_counts, np.uint32)
     V = views.shape[0]
     K = counts_in.shape[0]
-    cmds = np.zeros((V, stride), dtype=cmd_dtype)
-    counts = np.zeros((V, K), dtype=np.uint32)
+    if out is not None:
+        cmds, counts = out
+        assert cmds.shape == (V, stride) and cmds.flags.c_contiguous and counts.shape == (V, K)
+    else:
+        cmds = np.zeros((V, stride), dtype=cmd_dtype)
+        counts = np.zeros((V, K), dtype=np.uint32)
     lib().oracle_cull(_p(spheres), _p(np.ascontiguousarray(mesh_index, np.uint32)), _p(meshes), meshes.shape[0],
                       _p(counts_in), K, _p(views), V, stride, _p(cmds), _p(counts), nthreads)
     return cmds, counts

@@ -12,7 +12,12 @@
 //                     frames with scene edits between them (appends,
 //                     swap-removals, in-place and cross-StateSet
 //                     Drawable::create, StateSet edges), the resident
-//                     drawable list checked the same way every frame
+//                     drawable list checked the same way every frame;
+//                     then cull frames (recordDrawableCulling) with one and
+//                     with three views, with and without instanced
+//                     MatrixLists, some of them written on the GPU by a
+//                     transform level, checked against the CPU oracle
+//                     (oracle/cadr_oracle.h, linked by this test only)
 //
 // Like the reference's tests it prints nothing on success but "OK" and exits
 // non-zero with a message on the first failure.
@@ -22,6 +27,10 @@
 #include <cadr_hip.h>
 #include <hip/hip_runtime.h>
 
+#include "../../oracle/cadr_oracle.h"
+
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -243,6 +252,45 @@ static void removeChildOnce(StateSet& parent, StateSet& child)
 
 // tests/ParentChildTest.cpp: the childList / parentList operations, and the
 // relation seen from both ends.
+// Sustained MatrixList::setMatrices churn (every frame 100 lists get new
+// storage of 1..64 matrices, 2 GB over the run) with long-lived allocations
+// left behind in every block it touches: freed space inside a block is
+// reused (CircularAllocationMemory.h:522-560), blocks that empty go back,
+// so the number of memory blocks stays bounded.
+static void testAllocatorChurn()
+{
+	Renderer r(false);
+	DataStorage& ds = r.dataStorage();
+	std::mt19937 rng(11);
+	std::vector<std::unique_ptr<MatrixList>> lists;
+	for (int i = 0; i < 100; i++) lists.push_back(std::make_unique<MatrixList>(r));
+	std::vector<std::unique_ptr<DataAllocation>> keep;
+	std::vector<float> m(16 * 64, 1.f);
+	size_t maxBlocks = 0, allocatedBytes = 0;
+	for (int frame = 0; frame < 10000; frame++) {
+		for (auto& l : lists) {
+			const size_t n = 1 + rng() % 64;
+			l->setMatrices(m.data(), n);
+			allocatedBytes += 64 + 64 * n;
+		}
+		if (frame % 100 == 0) {  // a long-lived allocation wherever the allocator is at the moment
+			keep.push_back(std::make_unique<DataAllocation>(ds));
+			uint8_t bytes[256] = {};
+			keep.back()->upload(bytes, sizeof(bytes));
+		}
+		r.executeCopyOperations();
+		maxBlocks = std::max(maxBlocks, ds.numBlocks());
+		CHECK(ds.liveAllocations() == ds.handleTable().numTables() + lists.size() + keep.size(), "live allocation count");
+	}
+	CHECK(allocatedBytes > 2000000000ull, "the churn moves gigabytes");
+	CHECK(maxBlocks <= 4, "memory blocks stay bounded under churn: " + std::to_string(maxBlocks));
+	// everything freed: the storage drains to its first two blocks, reused from their starts
+	lists.clear();
+	keep.clear();
+	CHECK(ds.liveAllocations() == ds.handleTable().numTables(), "storage drained");
+	CHECK(ds.numBlocks() <= 2, "emptied blocks outside the allocation order go back");
+}
+
 static void testParentChildLists()
 {
 	Renderer r(false);
@@ -592,14 +640,288 @@ static void gpuResidentFrames(uint32_t seed)
 	(void)hipStreamDestroy(stream);
 }
 
+// ------------------------------------------------------------ cull frames
+
+// A column-major rotation (axis-angle) x uniform scale + translation.
+static void randomMatrix(std::mt19937& rng, float range, float* M)
+{
+	std::uniform_real_distribution<float> u(-1.f, 1.f), sc(0.5f, 2.f), ang(0.f, 6.2831853f);
+	float ax = u(rng), ay = u(rng), az = u(rng);
+	const float len = std::sqrt(ax * ax + ay * ay + az * az) + 1e-6f;
+	ax /= len; ay /= len; az /= len;
+	const float a = ang(rng), c = std::cos(a), s = std::sin(a), t = 1.f - c, k = sc(rng);
+	const float R[9] = {t * ax * ax + c,      t * ax * ay + s * az, t * ax * az - s * ay,
+	                    t * ax * ay - s * az, t * ay * ay + c,      t * ay * az + s * ax,
+	                    t * ax * az + s * ay, t * ay * az - s * ax, t * az * az + c};
+	for (int col = 0; col < 3; col++) {
+		for (int row = 0; row < 3; row++) M[4 * col + row] = k * R[3 * col + row];
+		M[4 * col + 3] = 0.f;
+	}
+	M[12] = range * u(rng); M[13] = range * u(rng); M[14] = range * u(rng); M[15] = 1.f;
+}
+
+// Cull frames through the facade against the oracle: drawables with one
+// MatrixList each (one cull item per drawable) or, with `instancing`, 0 to 3
+// matrices (one item per matrix, some lists written on the GPU by a
+// transform level); StateSets nested and one reached through two parents;
+// a Geometry destroyed while its drawables remain; views from cadrMakeView.
+static void gpuCullFrames(uint32_t seed, bool instancing, size_t numDrawables)
+{
+	std::printf("gpu cull frames: seed %u, instancing %d, %zu drawables\n", seed, int(instancing), numDrawables);
+	std::fflush(stdout);
+	Renderer r(0, false);
+	DataStorage& ds = r.dataStorage();
+	std::mt19937 rng(seed);
+	if (seed % 2) for (int i = 0; i < 3000; i++) ds.createHandle();  // handle level 2
+
+	const int G = 6;
+	std::vector<std::unique_ptr<Geometry>> geoms;
+	std::vector<std::vector<PrimitiveSet>> psData;
+	for (int g = 0; g <= G; g++) {  // geometry G is destroyed before the frames
+		geoms.push_back(std::make_unique<Geometry>(r));
+		std::vector<uint8_t> v(64), idx(64);
+		geoms.back()->uploadVertexData(v.data(), v.size());
+		geoms.back()->uploadIndexData(idx.data(), idx.size());
+		std::vector<PrimitiveSet> ps(5);  // LODs at offset 0 or 8, up to 4 of them
+		for (auto& p : ps) p = {uint32_t(36 + rng() % 3000), uint32_t(rng() % 1000000)};
+		geoms.back()->uploadPrimitiveSetData(ps.data(), ps.size() * sizeof(PrimitiveSet));
+		psData.push_back(ps);
+		std::uniform_real_distribution<float> u(-1.f, 1.f), rad(0.5f, 4.f);
+		geoms.back()->setBoundingSphere(BoundingSphere{{u(rng), u(rng), u(rng)}, rad(rng)});
+		geoms.back()->setNumLods(1 + uint32_t(g % 4));
+	}
+	const int K = 6;
+	std::vector<std::unique_ptr<StateSet>> ss;
+	for (int k = 0; k < K; k++) ss.push_back(std::make_unique<StateSet>(r));
+	ss[0]->childList.append(*ss[1]);
+	ss[0]->childList.append(*ss[2]);
+	ss[1]->childList.append(*ss[3]);
+	ss[2]->childList.append(*ss[3]);  // StateSet 3 is drawn twice, once per path
+	ss[2]->childList.append(*ss[4]);
+	ss[0]->childList.append(*ss[5]);
+
+	struct DInfo { int g; uint32_t off; size_t ml; };
+	std::vector<std::unique_ptr<MatrixList>> mls;
+	std::vector<std::vector<float>> mlHost;
+	std::vector<std::unique_ptr<Drawable>> drawables;
+	std::vector<DInfo> info;
+	for (size_t i = 0; i < numDrawables; i++) {
+		const size_t nm = instancing ? rng() % 4 : 1;
+		std::vector<float> m(16 * nm);
+		for (size_t t = 0; t < nm; t++) randomMatrix(rng, 300.f, &m[16 * t]);
+		mls.push_back(std::make_unique<MatrixList>(r));
+		mls.back()->setMatrices(m.data(), nm);
+		mlHost.push_back(m);
+		const int g = (i % 50 == 7) ? G : int(rng() % G);
+		const uint32_t off = 8u * uint32_t(rng() % 2);
+		drawables.push_back(std::make_unique<Drawable>(*geoms[g], off, *mls.back(), *ss[rng() % K]));
+		info.push_back({g, off, mls.size() - 1});
+	}
+	for (size_t i = 0; i < numDrawables / 20; i++) drawables[rng() % numDrawables]->destroy();
+	// the mesh table the GPU should build, before geometry G goes away
+	std::vector<cadrMesh> expMeshes(r.numMeshes());
+	for (int g = 0; g <= G; g++) {
+		for (uint32_t off : {0u, 8u}) {
+			const uint32_t m = r.meshIndexOf(*geoms[g], off);
+			if (m >= expMeshes.size()) expMeshes.resize(m + 1);
+			cadrMesh& e = expMeshes[m];
+			e = cadrMesh{};
+			if (g == G) {
+				e.radius = -INFINITY;
+				e.lodCount = 1;
+				continue;
+			}
+			const BoundingSphere& b = geoms[g]->boundingSphere();
+			e.center[0] = b.center[0]; e.center[1] = b.center[1]; e.center[2] = b.center[2]; e.radius = b.radius;
+			e.lodCount = std::min<uint32_t>(geoms[g]->numLods(), (40 - off) / 8);
+			for (uint32_t k = 0; k < e.lodCount; k++) {
+				e.indexCount[k] = psData[g][off / 8 + k].indexCount;
+				e.firstIndex[k] = psData[g][off / 8 + k].startIndex;
+			}
+		}
+	}
+	geoms[G].reset();
+	std::unordered_map<const Drawable*, size_t> indexOf;
+	for (size_t i = 0; i < drawables.size(); i++) indexOf[drawables[i].get()] = i;
+
+	// with instancing, the first lists with matrices get world = parent x local from a GPU transform level
+	std::vector<void*> deviceArrays;
+	auto devUpload = [&](const void* p, size_t bytes) {
+		void* d = nullptr;
+		if (hipMalloc(&d, std::max<size_t>(bytes, 16)) != hipSuccess) throw std::runtime_error("hipMalloc");
+		if (bytes && hipMemcpy(d, p, bytes, hipMemcpyHostToDevice) != hipSuccess) throw std::runtime_error("hipMemcpy");
+		deviceArrays.push_back(d);
+		return reinterpret_cast<uint64_t>(d);
+	};
+	std::vector<float> roots(16 * 4);
+	for (int i = 0; i < 4; i++) randomMatrix(rng, 100.f, &roots[16 * i]);
+	const uint64_t rootsDev = devUpload(roots.data(), roots.size() * 4);
+	struct GpuList { size_t ml; uint64_t local, parent; };
+	std::vector<GpuList> gpuLists;
+	if (instancing) {
+		for (size_t i = 0; i < mls.size() && gpuLists.size() < 16; i++) {
+			const size_t n = mls[i]->numMatrices();
+			if (n == 0) continue;
+			std::vector<float> local(16 * n);
+			std::vector<uint32_t> parent(n);
+			for (size_t t = 0; t < n; t++) {
+				randomMatrix(rng, 50.f, &local[16 * t]);
+				parent[t] = uint32_t(rng() % 4);
+				oracle_mat4_mul(&roots[16 * parent[t]], &local[16 * t], &mlHost[i][16 * t]);
+			}
+			gpuLists.push_back({i, devUpload(local.data(), local.size() * 4), devUpload(parent.data(), parent.size() * 4)});
+		}
+	}
+
+	std::vector<cadrView> views(3);
+	const float eyes[3][3] = {{0.f, 0.f, 0.f}, {350.f, 50.f, -200.f}, {-100.f, 300.f, 400.f}};
+	const float dirs[3][3] = {{0.f, 0.f, -1.f}, {-1.f, -0.1f, 0.6f}, {0.2f, -0.7f, -1.f}};
+	const float up[3] = {0.f, 1.f, 0.f};
+	const float fovs[3] = {1.0472f, 1.2f, 0.8f};
+	for (int v = 0; v < 3; v++)
+		CHECK(cadrMakeView(eyes[v], dirs[v], up, fovs[v], 16.f / 9.f, 0.1f, 2000.f, 1080.f, nullptr, &views[v]) == 0,
+		      "cadrMakeView");
+
+	hipStream_t stream;
+	if (hipStreamCreate(&stream) != hipSuccess) throw std::runtime_error("hipStreamCreate");
+	for (int frame = 0; frame < 3; frame++) {
+		const size_t V = frame == 0 ? 1 : 3;
+		r.setCullDrawablePointers(frame > 0);
+		if (frame == 2) {
+			// edits: a list changes size, a drawable moves StateSet, one is removed
+			std::vector<float> m(16 * 2);
+			randomMatrix(rng, 300.f, &m[0]);
+			randomMatrix(rng, 300.f, &m[16]);
+			const size_t target = instancing ? 3 : 5;
+			const size_t nm = instancing ? 2 : 1;
+			mls[target]->setMatrices(m.data(), nm);
+			mlHost[target].assign(m.begin(), m.begin() + 16 * nm);
+			for (GpuList& gl : gpuLists) if (gl.ml == target) gl.ml = size_t(-1);
+			for (size_t i = 0; i < drawables.size(); i++)
+				if (drawables[i]->isValid() && info[i].g != G) {
+					drawables[i]->create(*geoms[info[i].g], info[i].off, *mls[info[i].ml], *ss[(K - 1) - (i % 2)]);
+					break;
+				}
+			for (size_t i = drawables.size(); i-- > 0;)
+				if (drawables[i]->isValid()) { drawables[i]->destroy(); break; }
+		}
+		r.beginFrame();
+		r.executeCopyOperations();
+		for (const GpuList& gl : gpuLists) {
+			if (gl.ml == size_t(-1)) continue;
+			cadrTransformLevel lv{};
+			lv.localMatrices = gl.local;
+			lv.parentIndex = gl.parent;
+			lv.parentWorld = rootsDev;
+			lv.parentCount = 4;
+			lv.worldMatrices = mls[gl.ml]->matricesDeviceAddress();
+			lv.count = mls[gl.ml]->numMatrices();
+			CHECK(cadrPropagateTransforms(&lv, stream) == 0, "cadrPropagateTransforms into a MatrixList");
+		}
+		r.beginRecording(stream);
+		const size_t n = r.prepareSceneRendering(*ss[0]);
+		r.recordDrawableProcessing(stream, n);
+		r.recordDrawableCulling(stream, views.data(), V);
+		std::vector<DrawRange> ranges;
+		r.recordSceneRendering(stream, *ss[0], ranges);
+		r.endRecording(stream);
+		r.endFrame();
+		if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("hipStreamSynchronize");
+
+		// the items, their spheres and the mesh table the oracle expects
+		std::vector<float> spheres;
+		std::vector<uint32_t> meshIndex, rangeItems;
+		std::vector<std::pair<uint32_t, uint32_t>> source;
+		size_t slot = 0;
+		for (size_t k = 0; k < ranges.size(); k++) {
+			const DrawRange& range = ranges[k];
+			CHECK(range.itemBase == meshIndex.size(), "range item base");
+			for (size_t j = 0; j < range.count; j++, slot++) {
+				const DInfo& d = info[indexOf.at(&range.stateSet->getDrawable(j))];
+				const std::vector<float>& mats = mlHost[d.ml];
+				for (size_t t = 0; t < mats.size() / 16; t++) {
+					DrawableGpuData rec = range.stateSet->drawableDataList()[j];
+					const cadrMesh& me = expMeshes.at(rec.padding);
+					const float local[4] = {me.center[0], me.center[1], me.center[2], me.radius};
+					float out[4];
+					oracle_sphere_transform(&mats[16 * t], local, out);
+					spheres.insert(spheres.end(), out, out + 4);
+					meshIndex.push_back(rec.padding);
+					source.push_back({uint32_t(slot), uint32_t(t)});
+				}
+			}
+			CHECK(range.itemCount == meshIndex.size() - range.itemBase, "range item count");
+			rangeItems.push_back(uint32_t(range.itemCount));
+		}
+		const size_t items = meshIndex.size();
+		CHECK(r.numCullItems() == items && r.numCullViews() == V, "item and view counts");
+		std::vector<cadrMesh> gotMeshes(expMeshes.size());
+		ds.provider().download(gotMeshes.data(), r.cullMeshTableAddress(), gotMeshes.size() * sizeof(cadrMesh));
+		CHECK(std::memcmp(gotMeshes.data(), expMeshes.data(), gotMeshes.size() * sizeof(cadrMesh)) == 0,
+		      "mesh table built from the PrimitiveSets");
+		std::vector<float> gotSpheres(4 * items);
+		ds.provider().download(gotSpheres.data(), r.cullSphereBufferAddress(), gotSpheres.size() * 4);
+		CHECK(std::memcmp(gotSpheres.data(), spheres.data(), spheres.size() * 4) == 0, "item world spheres");
+
+		std::vector<cadrDrawIndexedIndirectCommand> expCmds(V * items);
+		std::vector<uint32_t> expCounts(V * ranges.size());
+		oracle_cull(spheres.data(), meshIndex.data(), expMeshes.data(), uint32_t(expMeshes.size()), rangeItems.data(),
+		            uint32_t(ranges.size()), views.data(), uint32_t(V), uint32_t(items), expCmds.data(),
+		            expCounts.data(), 1);
+		std::vector<uint32_t> gotCounts;
+		r.downloadCullCounts(gotCounts);
+		CHECK(gotCounts == expCounts, "per-view per-StateSet counts");
+		std::vector<cadrDrawIndexedIndirectCommand> gotCmds(V * items);
+		ds.provider().download(gotCmds.data(), r.cullCommandBufferAddress(), gotCmds.size() * 20);
+		std::vector<cadrDrawablePointers> gather(n), gotPtr;
+		ds.provider().download(gather.data(), r.drawablePointersBufferAddress(), n * 32);
+		if (frame > 0) {
+			gotPtr.resize(V * items);
+			ds.provider().download(gotPtr.data(), r.cullPointersBufferAddress(), gotPtr.size() * 32);
+		}
+		size_t visible = 0;
+		for (size_t w = 0; w < V; w++) {
+			for (size_t k = 0; k < ranges.size(); k++) {
+				for (uint32_t j = 0; j < expCounts[w * ranges.size() + k]; j++) {
+					const size_t at = w * items + ranges[k].itemBase + j;
+					cadrDrawIndexedIndirectCommand e = expCmds[at];
+					const auto src = source.at(e.firstInstance);
+					e.firstInstance = src.second;  // the matrix index of the item's MatrixList
+					CHECK(std::memcmp(&gotCmds[at], &e, 20) == 0,
+					      "command " + std::to_string(j) + " of view " + std::to_string(w) + " range " + std::to_string(k));
+					if (frame > 0)
+						CHECK(std::memcmp(&gotPtr[at], &gather[src.first], 32) == 0, "compacted DrawablePointers");
+					visible++;
+				}
+			}
+		}
+		CHECK(visible > 0 && visible < V * items, "some items culled, some visible");
+	}
+	(void)hipStreamDestroy(stream);
+	for (void* p : deviceArrays) (void)hipFree(p);
+	drawables.clear();
+	mls.clear();
+	geoms.clear();
+	ss.clear();
+}
+
 int main(int argc, char** argv)
 {
 	const std::string mode = argc > 1 ? argv[1] : "cpu";
 	try {
 		if (mode == "cpu") {
+			std::printf("handle levels\n");
+			std::fflush(stdout);
 			testHandleLevels();
+			std::printf("allocator placement\n");
+			std::fflush(stdout);
 			testAllocatorPlacement();
+			std::printf("allocator sequences\n");
+			std::fflush(stdout);
 			testAllocatorSequences();
+			std::printf("allocator churn\n");
+			std::fflush(stdout);
+			testAllocatorChurn();
 			testParentChildLists();
 			testStateSetOrder();
 		} else if (mode == "gpu") {
@@ -608,6 +930,8 @@ int main(int argc, char** argv)
 			gpuFrame(3, 20000, 3);
 			gpuResidentFrames(4);
 			gpuResidentFrames(5);
+			gpuCullFrames(6, false, 20000);
+			gpuCullFrames(7, true, 8000);
 		} else {
 			std::fprintf(stderr, "usage: facade_test cpu|gpu\n");
 			return 2;
