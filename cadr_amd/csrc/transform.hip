@@ -114,7 +114,16 @@ __global__ __launch_bounds__(256) void propagateBatchKernel(const cadrTransformL
 	uint32_t l = 0, cached = ~0u;
 	cadrTransformLevel L{};
 	for (uint64_t unit = uint64_t(blockIdx.x) * 4u + wave; unit < totalUnits; unit += waves) {
-		while (firstUnit[l + 1] <= unit) l++;  // units only grow, so the level index does too
+		// the level holding this unit: units only grow, so search upwards from
+		// the last one (a grid stride can skip many small levels)
+		if (firstUnit[l + 1] <= unit) {
+			uint32_t lo = l + 1, hi = numLevels - 1;
+			while (lo < hi) {
+				const uint32_t mid = (lo + hi + 1) / 2;
+				if (firstUnit[mid] <= unit) lo = mid; else hi = mid - 1;
+			}
+			l = lo;
+		}
 		if (l != cached) {  // the descriptor stays in scalar registers while the level lasts
 			L = levels[l];
 			cached = l;
