@@ -71,6 +71,7 @@ TILE_FIRST = 0x80000000
 TILE_LAST = 0x40000000
 TILE_STATESET_MASK = 0x3FFFFFFF
 CULL_NO_TILE_SPHERES = 1
+CULL_WHOLE_TILES = 2
 
 SUCCESS = 0
 ERROR_INVALID_VALUE = 1

@@ -79,6 +79,7 @@ struct CullParams {
 	uint32_t* ctl;
 	uint32_t numJobs;
 	uint32_t totalWork;
+	uint32_t splitTiles;  // batched: a job's last tiles handed out per view group
 	uint32_t flags;
 	uint64_t waitLimit;                 // steady-counter ticks a look-back may wait
 	LeafParams leaf;                    // one-view fused kernel
@@ -569,11 +570,13 @@ __device__ __forceinline__ void issueItemLoads(const LeafParams& Lf, uint64_t fi
 	}
 }
 
+// store: write the world matrices (and spheres); a tile that several work
+// items of one launch share is written by one of them.
 template <int ITEMS, bool LDS, bool PREFETCH = true>
 __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const cadrCullJob& J, const TileDesc& T,
                                                    uint32_t base, uint32_t lane, float4* stage,
                                                    const MeshTable& meshes, float4 (&sp)[ITEMS],
-                                                   uint32_t (&mi)[ITEMS])
+                                                   uint32_t (&mi)[ITEMS], bool store = true)
 {
 	const float4 empty = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
 	if (T.count == 0) {
@@ -638,10 +641,10 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 #pragma unroll
 			for (int c = 0; c < 4; c++) W[c] = L[c];
 		}
-		storeMatrices(world, first, end, stage, lane, W);
+		if (store) storeMatrices(world, first, end, stage, lane, W);
 		float4 s = empty;
 		if (live && mi[j] < J.numMeshes) s = sphereTransform(W, meshSphere<LDS>(meshes, mi[j]));
-		if (Lf.spheres && live) storeF4(globalMut<float4>(Lf.spheres) + i, s);
+		if (store && Lf.spheres && live) storeF4(globalMut<float4>(Lf.spheres) + i, s);
 		sp[j] = s;
 		// without the prefetch, keep the scheduler from hoisting later items'
 		// loads above this one (every item's 32 registers of loads at once)
@@ -833,7 +836,21 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			job = uniform(lo);
 		}
 		const cadrCullJob J = loadJob(P.jobs, job);
-		const uint32_t t = w - constPtr<uint32_t>(addr(P.firstWork))[job];
+		// the work item: a whole tile for every view, or, among the job's last
+		// split tiles, one tile for one group of views (tile-major, so a
+		// tile's look-back predecessors still hold earlier tickets)
+		const uint32_t u = w - constPtr<uint32_t>(addr(P.firstWork))[job];
+		const uint32_t groups = J.numViews / kViewGroup + (J.numViews % kViewGroup != 0u ? 1u : 0u);
+		const uint32_t split = groups > 1 ? min(J.numTiles, P.splitTiles) : 0u;
+		const uint32_t whole = J.numTiles - split;
+		uint32_t t = u, gBegin = 0, gEnd = J.numViews;
+		if (u >= whole) {
+			const uint32_t q = u - whole;
+			t = uniform(whole + q / groups);
+			gBegin = uniform((q % groups) * kViewGroup);
+			gEnd = gBegin + min(J.numViews - gBegin, kViewGroup);
+		}
+		const bool storeLeaf = gBegin == 0;
 		const TileDesc T = loadTile(J, t, S::TILE);
 		const bool ldsMeshes = stageMeshes<BLOCK>(sh.meshes, J, tid);
 
@@ -850,9 +867,11 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			constexpr bool PREFETCH = ITEMS >= 8;
 			float4* stage = &sMat[wave][0];
 			if (ldsMeshes)
-				propagateTileItems<ITEMS, true, PREFETCH>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp, mi);
+				propagateTileItems<ITEMS, true, PREFETCH>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp, mi,
+				                                          storeLeaf);
 			else
-				propagateTileItems<ITEMS, false, PREFETCH>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp, mi);
+				propagateTileItems<ITEMS, false, PREFETCH>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp, mi,
+				                                           storeLeaf);
 		} else {
 			loadTileItems<ITEMS>(globalPtr<float4>(J.spheres), globalPtr<uint32_t>(J.meshIndex), T,
 			                     wave * WAVE_ITEMS + lane, sp, mi);
@@ -950,7 +969,7 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		const uint64_t views = J.views;
 		uint64_t* chain0 = P.status + constPtr<uint64_t>(addr(P.statusBase))[job];
 
-		for (uint32_t g0 = 0; g0 < J.numViews; g0 += kViewGroup) {
+		for (uint32_t g0 = gBegin; g0 < gEnd; g0 += kViewGroup) {
 			const uint32_t vg = uniform(min(kViewGroup, J.numViews - g0));
 
 			// tile classes of the group's views, lane v holding view g0+v, and
@@ -1118,10 +1137,20 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 	}
 }
 
+// Work items of a job in a batched launch: its tiles, plus one more per
+// extra view group for each of its last splitTiles tiles (the kernel's rule).
+// The host checks that a launch's total fits 32 bits.
+__host__ __device__ __forceinline__ uint64_t jobWorkItems(uint32_t numTiles, uint32_t numViews, uint32_t splitTiles)
+{
+	const uint64_t groups = (uint64_t(numViews) + kViewGroup - 1) / kViewGroup;
+	const uint64_t split = groups > 1 ? (numTiles < splitTiles ? numTiles : splitTiles) : 0u;
+	return numTiles + split * (groups - 1);
+}
+
 // Job offsets for a batched launch (prefix sums over jobs), ticket reset and
 // a new epoch; one block, before the cull kernel on the same stream.
 __global__ __launch_bounds__(256) void cullSetupKernel(const cadrCullJob* __restrict__ jobs, uint32_t numJobs,
-                                                       uint32_t* __restrict__ firstWork,
+                                                       uint32_t splitTiles, uint32_t* __restrict__ firstWork,
                                                        uint64_t* __restrict__ statusBase, uint32_t* __restrict__ ctl)
 {
 	__shared__ uint32_t sT[256];
@@ -1134,8 +1163,8 @@ __global__ __launch_bounds__(256) void cullSetupKernel(const cadrCullJob* __rest
 		uint32_t nt = 0;
 		uint64_t ns = 0;
 		if (j < numJobs) {
-			nt = jobs[j].numTiles;
-			ns = uint64_t(jobs[j].numViews) * nt;
+			nt = uint32_t(jobWorkItems(jobs[j].numTiles, jobs[j].numViews, splitTiles));
+			ns = uint64_t(jobs[j].numViews) * jobs[j].numTiles;
 		}
 		sT[tid] = nt;
 		sS[tid] = ns;
@@ -1381,11 +1410,18 @@ int launchBatch(KernelFn fn, const KernelEntry& ke, const cadrCullConfig* config
 	CADR_REQUIRE(deviceJobs && hostJobs, name + ": null job array");
 	CADR_REQUIRE(deviceJobs % 16 == 0, name + ": deviceJobs must be 16-byte aligned");
 	CADR_REQUIRE(numJobs <= ws->maxJobs, name + ": more jobs than the workspace was created for");
+	// the persistent grid; unless CADR_CULL_WHOLE_TILES, a job's last grid
+	// tiles are handed out per view group
+	const int cu = computeUnitCount();
+	uint32_t perCU = config->workgroupsPerCU;
+	if (perCU == 0) perCU = ke.block >= 512 ? 2 : 4;
+	uint64_t grid = uint64_t(cu > 0 ? cu : 256) * perCU;
+	const uint32_t splitTiles = (config->flags & CADR_CULL_WHOLE_TILES) ? 0u : uint32_t(std::min<uint64_t>(grid, 1u << 20));
 	uint64_t totalWork = 0, statusNeed = 0;
 	for (uint32_t j = 0; j < numJobs; j++) {
 		const int rc = checkJob(hostJobs[j]);
 		if (rc != CADR_SUCCESS) return rc;
-		totalWork += hostJobs[j].numTiles;
+		totalWork += jobWorkItems(hostJobs[j].numTiles, hostJobs[j].numViews, splitTiles);
 		statusNeed += uint64_t(hostJobs[j].numViews) * hostJobs[j].numTiles;
 	}
 	CADR_REQUIRE(totalWork < 0xffffffffull, name + ": too many tiles in one launch");
@@ -1396,15 +1432,10 @@ int launchBatch(KernelFn fn, const KernelEntry& ke, const cadrCullConfig* config
 
 	hipStream_t s = hipStream_t(stream);
 	const auto* jobs = reinterpret_cast<const cadrCullJob*>(deviceJobs);
-	hipLaunchKernelGGL(cullSetupKernel, dim3(1), dim3(256), 0, s, jobs, numJobs, ws->firstWork, ws->statusBase,
-	                   ws->ctl);
+	hipLaunchKernelGGL(cullSetupKernel, dim3(1), dim3(256), 0, s, jobs, numJobs, splitTiles, ws->firstWork,
+	                   ws->statusBase, ws->ctl);
 	CADR_CHECK_HIP(hipGetLastError(), "cullSetupKernel launch");
 	if (totalWork == 0) return ok();
-
-	const int cu = computeUnitCount();
-	uint32_t perCU = config->workgroupsPerCU;
-	if (perCU == 0) perCU = ke.block >= 512 ? 2 : 4;
-	uint64_t grid = uint64_t(cu > 0 ? cu : 256) * perCU;
 	if (grid > totalWork) grid = totalWork;
 	CullParams p{};
 	p.jobs = jobs;
@@ -1414,6 +1445,7 @@ int launchBatch(KernelFn fn, const KernelEntry& ke, const cadrCullConfig* config
 	p.ctl = ws->ctl;
 	p.numJobs = numJobs;
 	p.totalWork = uint32_t(totalWork);
+	p.splitTiles = splitTiles;
 	p.flags = config->flags;
 	p.waitLimit = ws->waitLimit;
 	p.leaves = leaves;

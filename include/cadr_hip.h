@@ -366,9 +366,15 @@ typedef struct cadrCullConfig {
 	uint32_t blockSize;         /* 256 or 512 threads                          */
 	uint32_t itemsPerThread;    /* 2, 4, 8 or 16                               */
 	uint32_t workgroupsPerCU;   /* persistent grid = CUs * this (0 = default)  */
-	uint32_t flags;             /* CADR_CULL_NO_TILE_SPHERES: test every drawable */
+	uint32_t flags;             /* CADR_CULL_* below                           */
 } cadrCullConfig;
+/* Test every drawable (no per-tile bounding sphere classification). */
 #define CADR_CULL_NO_TILE_SPHERES 1u
+/* Batched launches: every work item is one tile for all of its job's views.
+ * By default the last (persistent grid size) tiles of a job with more than
+ * 64 views are handed out as one work item per (tile, 64-view group), so the
+ * launch's last work items are short and its workgroups finish together. */
+#define CADR_CULL_WHOLE_TILES 2u
 
 /* Device workspace of the cull launches: look-back status words, tile
  * tickets, the launch epoch and per-job offsets.  One workspace serves one

@@ -291,6 +291,36 @@ static void testAllocatorChurn()
 	CHECK(ds.numBlocks() <= 2, "emptied blocks outside the allocation order go back");
 }
 
+// Renderer::prepareSceneRendering grows the four per-frame buffers to
+// 1.2 N slots, at least 128, and only when N exceeds the current capacity
+// (Renderer.cpp:461-470): values worked by hand.
+static void testBufferGrowth()
+{
+	Renderer r(false);
+	Geometry g(r);
+	MatrixList ml(r);
+	const float id[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+	ml.setMatrices(id, 1);
+	StateSet root(r), child(r);
+	root.childList.append(child);
+	std::vector<Drawable> ds;
+	ds.reserve(2000);
+	auto grow = [&](size_t n) {
+		while (ds.size() < n) ds.emplace_back(g, 0u, ml, ds.size() % 2 ? child : root);
+		return r.prepareSceneRendering(root);
+	};
+	CHECK(grow(0) == 0 && r.bufferCapacity() == 0, "nothing to draw: no buffers");
+	CHECK(grow(1) == 1 && r.bufferCapacity() == 128, "at least 128 slots");
+	CHECK(grow(128) == 128 && r.bufferCapacity() == 128, "no growth while N fits");
+	CHECK(grow(129) == 129 && r.bufferCapacity() == 154, "1.2 x 129 = 154.8 -> 154 slots");
+	CHECK(grow(154) == 154 && r.bufferCapacity() == 154, "no growth at the capacity");
+	CHECK(grow(155) == 155 && r.bufferCapacity() == 186, "1.2 x 155 = 186 slots");
+	CHECK(grow(1000) == 1000 && r.bufferCapacity() == 1200, "1.2 x 1000 = 1200 slots");
+	ds.resize(10);
+	CHECK(r.prepareSceneRendering(root) == 10 && r.bufferCapacity() == 1200, "buffers never shrink");
+	ds.clear();
+}
+
 static void testParentChildLists()
 {
 	Renderer r(false);
@@ -922,6 +952,7 @@ int main(int argc, char** argv)
 			std::printf("allocator churn\n");
 			std::fflush(stdout);
 			testAllocatorChurn();
+			testBufferGrowth();
 			testParentChildLists();
 			testStateSetOrder();
 		} else if (mode == "gpu") {

@@ -119,10 +119,15 @@ def test_cull_many_views_and_statesets(gpu):
     exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
                                        cmd_dtype=N.CMD_DTYPE, nthreads=8)
     assert exp_counts.sum() > 10000
-    for block, items, flags in [(256, 4, 0), (256, 8, 0), (512, 2, N.CULL_NO_TILE_SPHERES)]:
-        print(f"300 views block={block} items={items} flags={flags}", flush=True)
-        cmds, counts = _run(ds, views, cull.make_config(block, items, flags=flags))
-        _compare(cmds, counts, exp_cmds, exp_counts, scene.stateset_counts, f"{block}x{items}")
+    # a job's last (persistent grid) tiles run as one work item per 64-view
+    # group: every tile at the default grid; 256 x 2 with one workgroup per CU
+    # splits only the last 256 of its ~450 tiles; CULL_WHOLE_TILES none
+    for block, items, per_cu, flags in [(256, 4, 0, 0), (256, 8, 0, 0), (512, 2, 0, N.CULL_NO_TILE_SPHERES),
+                                        (256, 2, 1, 0), (256, 2, 1, N.CULL_WHOLE_TILES),
+                                        (256, 8, 0, N.CULL_WHOLE_TILES)]:
+        print(f"300 views block={block} items={items} per_cu={per_cu} flags={flags}", flush=True)
+        cmds, counts = _run(ds, views, cull.make_config(block, items, per_cu, flags=flags))
+        _compare(cmds, counts, exp_cmds, exp_counts, scene.stateset_counts, f"{block}x{items} flags {flags}")
     ds.close()
 
 
