@@ -251,7 +251,7 @@ def main():
         stream.synchronize()
         device.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / args.steps
-        return ms, t_leaf.mean_ms(), (None if fused else t_cull.mean_ms()), out1.download()[1].copy()
+        return ms, t_leaf.mean_ms(), (None if fused else t_cull.mean_ms()), out1.download_counts()
 
     un_ms, leaf_ms, cull_ms, counts_un = run_frames(False)
     ms_per_step, fused_ms, _, counts1 = run_frames(True)
@@ -476,7 +476,7 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device, p
     plain_ms, plain_mean_ms, counts_plain = None, None, None
     if not args.no_plain:
         plain_ms, plain_mean_ms = time_plain(run_plain, stream, max(2, args.batch_steps // 2))
-        _, counts_plain = out.download()
+        counts_plain = out.download_counts()
 
     # batched, captured once into a HIP graph and replayed
     graph = Graph()
@@ -495,7 +495,7 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device, p
     stream.synchronize()
     batch_ms = 1e3 * (time.perf_counter() - t0) / args.batch_steps
     batch_kernel_ms = e0.elapsed_ms(e1) / args.batch_steps
-    _, counts = out.download()
+    counts = out.download_counts()
     same = None if counts_plain is None else bool(np.array_equal(counts, counts_plain))
     survivors = int(counts.sum())
     tests = V * n
@@ -561,7 +561,7 @@ def batched_frames(args, entries, leaf_descs, propagate_inner, batch, stream, li
     unfused = unfused or separate
 
     un_ms = time_graph(unfused, stream, steps)
-    counts_un = [e[2].download()[1].copy() for e in entries]
+    counts_un = [e[2].download_counts() for e in entries]
     fused_leaves = leaves.copy()
     fused_leaves["spheres"] = 0
     sweep, same = {}, True
@@ -571,7 +571,7 @@ def batched_frames(args, entries, leaf_descs, propagate_inner, batch, stream, li
         for e in entries:
             e[2].counts.upload(np.zeros(e[0].K * e[1].shape[0], np.uint32))
         sweep[f"{block}x{items}"] = time_graph(lambda: (propagate_inner(), b.run_fused_batch(s)), stream, steps)
-        same = same and all(np.array_equal(e[2].download()[1], c) for e, c in zip(entries, counts_un))
+        same = same and all(np.array_equal(e[2].download_counts(), c) for e, c in zip(entries, counts_un))
         b.close()
     best = min(sweep, key=sweep.get)
     # the fused launch alone (the levels above are already propagated), for its roofline
@@ -683,7 +683,7 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     plain_ms, plain_mean_ms, counts_plain = None, None, None
     if not args.no_plain:
         plain_ms, plain_mean_ms = time_plain(run_plain, stream, 3)
-        counts_plain = [e[2].download()[1] for e in entries]
+        counts_plain = [e[2].download_counts() for e in entries]
 
     frame = Graph()
     frame.begin(s)
@@ -711,7 +711,7 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     e1.record(s)
     stream.synchronize()
     cull_ms = e0.elapsed_ms(e1) / steps
-    counts = [e[2].download()[1] for e in entries]
+    counts = [e[2].download_counts() for e in entries]
     same = None if counts_plain is None else all(np.array_equal(a, b) for a, b in zip(counts, counts_plain))
     survivors = int(sum(c.sum() for c in counts))
     tests = sum(ds.n * v.shape[0] for ds, v in zip(dss, views_l))
@@ -1118,7 +1118,7 @@ def bench_plain100k(args, stream, lib, N, cull, scenes, device):
     stream.synchronize()
     back_to_back = e0.elapsed_ms(e1) / 200
     graph_ms = time_graph(lambda: [b.run_view(s) for _ in range(100)], stream, 5) / 100
-    counts = out.download()[1]
+    counts = out.download_counts()
     survivors = int(counts.sum())
     _, spheres = oracle.propagate_scene(scene)
     exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,

@@ -189,8 +189,12 @@ class CullOutputs:
 
     def download(self):
         cmds = self.commands.download(N.CMD_DTYPE, self.num_views * self.stride).reshape(self.num_views, self.stride)
-        counts = self.counts.download(np.uint32, self.num_views * self.K).reshape(self.num_views, self.K)
-        return cmds, counts
+        return cmds, self.download_counts()
+
+    def download_counts(self):
+        """The per-view StateSet counts alone (the commands of many views can
+        be tens of GB)."""
+        return self.counts.download(np.uint32, self.num_views * self.K).reshape(self.num_views, self.K)
 
     def close(self):
         self.commands.close(); self.counts.close()

@@ -20,15 +20,16 @@ def test_traffic_recorded_for_every_fused_shape():
     for shape in ("512,4", "512,2", "1024,1", "1024,2", "256,2", "256,4", "256,8", "256,16"):
         t = b.measured_traffic(f"cullFusedKernel<{shape}>")
         assert t is not None and 1.3e9 < t < 1.7e9, shape  # about 1.05x the 1.39 GB algorithmic
-    assert b.measured_traffic("cullKernel<256,8>") == 8925500000
+    assert b.measured_traffic("cullKernel<256,8>") == 9009600000
+    assert b.measured_traffic("cullKernel<512,4>") == 8917600000
     assert b.measured_traffic("processDrawablesKernel<3>") == 2330500000
     assert b.measured_traffic("noSuchKernel<1,1>") is None
 
 
 def test_valu_recorded_for_the_batched_shapes():
     b = _bench()
-    assert b.measured_per_launch("valu_wave_instructions", "cullKernel<256,8>") == 637993224
-    assert b.measured_per_launch("valu_wave_instructions", "cullKernel<512,4>") == 681298778
-    v = b.valu_issue("cullKernel<256,8>", 1.866, 256)
-    assert abs(v["issue_frac"] - 637993224 / (256 * 2.4e9 * 1.866e-3)) < 1e-12
+    assert b.measured_per_launch("valu_wave_instructions", "cullKernel<256,8>") == 647785236
+    assert b.measured_per_launch("valu_wave_instructions", "cullKernel<512,4>") == 687555878
+    v = b.valu_issue("cullKernel<256,8>", 1.81, 256)
+    assert abs(v["issue_frac"] - 647785236 / (256 * 2.4e9 * 1.81e-3)) < 1e-12
     assert b.valu_issue("cullKernel<128,8>", 2.0, 256) is None
