@@ -88,6 +88,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--block", type=int, default=512, help="batched cull workgroup size (with --no-tune)")
     p.add_argument("--items", type=int, default=4, help="batched cull drawables per thread (with --no-tune)")
+    p.add_argument("--view-block", type=int, default=256, help="single-view cull workgroup size (with --no-tune)")
     p.add_argument("--view-items", type=int, default=16, help="single-view cull drawables per thread (with --no-tune)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="also copy BYTES device-to-device 3 times (reference kernel for HBM counter scaling)")
@@ -162,7 +163,7 @@ def main():
     ds = cull.DeviceScene(spec)  # streamed to HBM level by level
     log(f"scene generated and uploaded in {time.time() - t0:.1f}s: {ds.n} drawables, {sum(spec.sizes)} nodes")
     n = ds.n
-    config = cull.make_config(256, args.view_items)
+    config = cull.make_config(args.view_block, args.view_items)
     views1 = cull.make_views([scenes.DEFAULT_CAMERA])
     out1 = cull.CullOutputs(1, n, ds.K)
     tuned1 = None
@@ -196,7 +197,7 @@ def main():
         for lv in upper:
             N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
 
-    # kernel shape of the fused leaf + cull launch: the fastest of the four
+    # kernel shape of the fused leaf + cull launch: the fastest of the sweep
     fused_config, fused_table = cull.make_config(256, args.view_items), None
     if not args.no_tune:
         propagate_inner()
@@ -483,7 +484,7 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device, p
     graph.begin(s)
     batch.run(s)
     graph.end(s)
-    for _ in range(2):
+    for _ in range(max(2, args.warmup)):  # the clocks settle after the latency-bound plain pass
         graph.launch(s)
     stream.synchronize()
     e0, e1 = Event(), Event()

@@ -772,10 +772,10 @@ __device__ __forceinline__ void cullViewBody(const CullParams& P)
 	}
 }
 
-template <int ITEMS>
-__global__ __launch_bounds__(256) void cullViewKernel(CullParams P)
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void cullViewKernel(CullParams P)
 {
-	cullViewBody<256, ITEMS, false>(P);
+	cullViewBody<BLOCK, ITEMS, false>(P);
 }
 
 // The fused kernel is latency-bound (a chain of dependent loads per item), so
@@ -1217,20 +1217,25 @@ using KernelFn = void (*)(CullParams);
 struct KernelEntry {
 	uint32_t block, items;
 	KernelFn batch;       // many views / jobs
-	KernelFn view;        // one view (256-thread blocks only), or nullptr
+	KernelFn view;        // one view, or nullptr
 	KernelFn fused;       // leaf propagation + one view, or nullptr
 	KernelFn fusedBatch;  // leaf propagation + many views / jobs, or nullptr
 };
 
 const KernelEntry kKernels[] = {
-    {256, 2, cullKernel<256, 2, false>, cullViewKernel<2>, cullFusedKernel<256, 2, 4>, nullptr},
-    {256, 4, cullKernel<256, 4, false>, cullViewKernel<4>, cullFusedKernel<256, 4, 4>, cullKernel<256, 4, true>},
-    {256, 8, cullKernel<256, 8, false>, cullViewKernel<8>, cullFusedKernel<256, 8, 1>, cullKernel<256, 8, true>},
-    {256, 16, cullKernel<256, 16, false>, cullViewKernel<16>, cullFusedKernel<256, 16, 1>, nullptr},
-    {512, 2, cullKernel<512, 2, false>, nullptr, cullFusedKernel<512, 2, 1>, nullptr},
-    {512, 4, cullKernel<512, 4, false>, nullptr, cullFusedKernel<512, 4, 1>, cullKernel<512, 4, true>},
-    {1024, 1, nullptr, nullptr, cullFusedKernel<1024, 1, 1>, nullptr},
-    {1024, 2, nullptr, nullptr, cullFusedKernel<1024, 2, 1>, nullptr},
+    {256, 2, cullKernel<256, 2, false>, cullViewKernel<256, 2>, cullFusedKernel<256, 2, 4>, nullptr},
+    {256, 4, cullKernel<256, 4, false>, cullViewKernel<256, 4>, cullFusedKernel<256, 4, 4>, cullKernel<256, 4, true>},
+    {256, 8, cullKernel<256, 8, false>, cullViewKernel<256, 8>, cullFusedKernel<256, 8, 1>, cullKernel<256, 8, true>},
+    {256, 16, cullKernel<256, 16, false>, cullViewKernel<256, 16>, cullFusedKernel<256, 16, 1>, nullptr},
+    {512, 2, cullKernel<512, 2, false>, cullViewKernel<512, 2>, cullFusedKernel<512, 2, 1>, nullptr},
+    {512, 4, cullKernel<512, 4, false>, cullViewKernel<512, 4>, cullFusedKernel<512, 4, 1>, cullKernel<512, 4, true>},
+    {512, 8, nullptr, cullViewKernel<512, 8>, nullptr, nullptr},
+    {512, 16, nullptr, cullViewKernel<512, 16>, nullptr, nullptr},
+    {1024, 1, nullptr, cullViewKernel<1024, 1>, cullFusedKernel<1024, 1, 1>, nullptr},
+    {1024, 2, nullptr, cullViewKernel<1024, 2>, cullFusedKernel<1024, 2, 1>, nullptr},
+    {1024, 4, nullptr, cullViewKernel<1024, 4>, nullptr, nullptr},
+    {1024, 8, nullptr, cullViewKernel<1024, 8>, nullptr, nullptr},
+    {1024, 16, nullptr, cullViewKernel<1024, 16>, nullptr, nullptr},
     {128, 8, cullKernel<128, 8, false>, nullptr, nullptr, nullptr},
     {128, 16, cullKernel<128, 16, false>, nullptr, nullptr, nullptr},
 };
@@ -1242,6 +1247,23 @@ const KernelEntry* findKernel(const cadrCullConfig* c)
 		if (k.block == c->blockSize && k.items == c->itemsPerThread) return &k;
 	return nullptr;
 }
+
+}  // namespace
+
+bool cadr::cullHasKernel(const cadrCullConfig* c, CullPath path)
+{
+	const KernelEntry* k = findKernel(c);
+	if (!k) return false;
+	switch (path) {
+	case CullPath::View: return k->view != nullptr;
+	case CullPath::Batch: return k->batch != nullptr;
+	case CullPath::Fused: return k->fused != nullptr;
+	case CullPath::FusedBatch: return k->fusedBatch != nullptr;
+	}
+	return false;
+}
+
+namespace {
 
 int checkJob(const cadrCullJob& J)
 {
@@ -1506,7 +1528,7 @@ CADR_API int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, cons
 	p.totalWork = hostJob->numTiles;
 	p.flags = config->flags;
 	p.waitLimit = ws->waitLimit;
-	hipLaunchKernelGGL(ke->view, dim3(hostJob->numTiles), dim3(256), 0, hipStream_t(stream), p);
+	hipLaunchKernelGGL(ke->view, dim3(hostJob->numTiles), dim3(ke->block), 0, hipStream_t(stream), p);
 	CADR_CHECK_HIP(hipGetLastError(), "cullViewKernel launch");
 	return ok();
 }

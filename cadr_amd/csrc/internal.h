@@ -18,6 +18,10 @@ inline int ok() { return CADR_SUCCESS; }
 // Compute-unit count of the current device (cached per device).
 int computeUnitCount();
 
+// Whether the cull entry point of `path` has a kernel for this shape.
+enum class CullPath { View, Batch, Fused, FusedBatch };
+bool cullHasKernel(const cadrCullConfig* c, CullPath path);
+
 }  // namespace cadr
 
 #define CADR_CHECK_HIP(expr, what)                                  \

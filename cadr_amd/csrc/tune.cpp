@@ -100,11 +100,10 @@ CADR_API int cadrCullAutotune(const cadrCullTuneJob* jobs, uint32_t numJobs, uin
 	if (candidates && numCandidates) {
 		cands.assign(candidates, candidates + numCandidates);
 	} else {
-		for (uint32_t block : {128u, 256u, 512u})
-			for (uint32_t items : {2u, 4u, 8u, 16u}) {
+		for (uint32_t block : {128u, 256u, 512u, 1024u})
+			for (uint32_t items : {1u, 2u, 4u, 8u, 16u}) {
 				const cadrCullConfig c{block, items, 0, 0};
-				if (cadrCullTileSize(&c) == 0) continue;
-				if (single && block != 256u) continue;  // the one-view kernel runs 256-thread workgroups
+				if (!cullHasKernel(&c, single ? CullPath::View : CullPath::Batch)) continue;
 				cands.push_back(c);
 				if (!single) cands.push_back(cadrCullConfig{block, items, 0, CADR_CULL_NO_TILE_SPHERES});
 			}

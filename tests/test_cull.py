@@ -98,6 +98,23 @@ def test_cull_plain100k_single_view(plain, block, items, flags, path):
     _compare(cmds, counts, exp_cmds, exp_counts, scene.stateset_counts, "plain100k")
 
 
+@pytest.mark.parametrize("block,items", [(512, 8), (512, 16), (1024, 1), (1024, 2), (1024, 4), (1024, 8),
+                                         (1024, 16)])
+def test_cull_one_view_large_workgroups(plain, block, items):
+    """The one-view kernel at 512 and 1024 threads (tiles up to 16384
+    drawables), against the oracle; shapes with no batched kernel are
+    refused by the batched entry point."""
+    scene, ds, spheres = plain
+    print(f"plain100k view block={block} items={items}", flush=True)
+    views = cull.make_views([scenes.DEFAULT_CAMERA])
+    cmds, counts = _run(ds, views, cull.make_config(block, items), single=True)
+    exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
+                                       cmd_dtype=N.CMD_DTYPE)
+    _compare(cmds, counts, exp_cmds, exp_counts, scene.stateset_counts, f"plain100k {block}x{items}")
+    with pytest.raises(N.CadrError):
+        _run(ds, views, cull.make_config(block, items))
+
+
 def test_cull_views_match_oracle_views(plain):
     cams = scenes.batch_cameras(64)
     for c in cams:

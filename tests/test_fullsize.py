@@ -75,19 +75,22 @@ def test_full10m_unfused_and_fused_frames(full10m):
     assert exp_counts.sum() > 1_000_000
 
     # unfused: every level on its own (leaf level writes world spheres), then the one-view cull
-    _log("full10m unfused: propagation by level + cullViewKernel<16>")
+    _log("full10m unfused: propagation by level + cullViewKernel<256,16> and <1024,8>")
     ds.propagate()
     out = cull.CullOutputs(1, ds.n, ds.K)
-    b = cull.CullBatch(cull.make_config(256, 16), [(ds, views, out)])
-    b.run_view()
     synchronize()
-    b.workspace.check()
     assert ds.download_spheres().tobytes() == spheres.tobytes(), "world spheres differ"
     for d in range(3):
         assert ds.world_matrices(d).tobytes() == worlds[d].tobytes(), f"level {d} world matrices differ"
-    cmds, counts = out.download()
-    _check_view(cmds[0], counts[0], exp_cmds[0], exp_counts[0], bases, "full10m unfused")
-    b.close()
+    for shape in ((256, 16), (1024, 8)):
+        out.commands.fill_u32(0)
+        b = cull.CullBatch(cull.make_config(*shape), [(ds, views, out)])
+        b.run_view()
+        synchronize()
+        b.workspace.check()
+        cmds, counts = out.download()
+        _check_view(cmds[0], counts[0], exp_cmds[0], exp_counts[0], bases, f"full10m unfused {shape}")
+        b.close()
 
     # fused: root and inner level, then the leaf level fused with the cull (the headline frame's launches)
     for shape in ((512, 4), (256, 8)):

@@ -39,7 +39,12 @@ def test_autotune_batched_and_single_view(gpu):
     one = views[3:4]
     out1 = cull.CullOutputs(1, ds.n, ds.K)
     best1, table1 = autotune([(ds, one, out1)], single_view=True, reps=2)
-    assert len(table1) == 4 and all(t["agrees"] for t in table1) and best1.blockSize == 256
+    # the one-view kernel at 256, 512 and 1024 threads
+    assert len(table1) == 13 and all(t["agrees"] for t in table1)
+    assert {(t["block"], t["items"]) for t in table1} == {(b, i) for b in (256, 512) for i in (2, 4, 8, 16)} | {
+        (1024, i) for i in (1, 2, 4, 8, 16)}
+    fastest1 = min(table1, key=lambda t: t["ms"])
+    assert (best1.blockSize, best1.itemsPerThread) == (fastest1["block"], fastest1["items"])
     _compare(out1, sc, sph, one)
 
     # explicit candidates, and an unsupported one is refused
