@@ -226,7 +226,7 @@ private:
 	uint32_t _cullTileSize = 0;
 	uint32_t _cullNumTiles = 0;
 	bool _cullPointersOn = false;
-	cadrCullConfig _cullSingle{256, 16, 0, 0}, _cullBatched{256, 8, 0, 0};
+	cadrCullConfig _cullSingle{1024, 8, 0, 0}, _cullBatched{256, 8, 0, 0};
 	struct CullBuffers {
 		uint64_t spheres = 0, meshIndex = 0, itemBase = 0, itemSource = 0, rangeBase = 0, tiles = 0;
 		uint64_t views = 0, job = 0, commands = 0, counts = 0, pointers = 0, sources = 0, meshes = 0;
