@@ -163,6 +163,9 @@ SIGNATURES = {
     "cadrScatterDrawables": (c_int, [c_u64, c_u64, c_u64, c_u64, c_u32, c_void_p]),
     "cadrPropagateTransforms": (c_int, [c_void_p, c_void_p]),
     "cadrPropagateTransformsBatch": (c_int, [c_u64, c_void_p, c_u32, c_void_p]),
+    "cadrTransformPlanCreate": (c_int, [c_void_p, c_u32, c_void_p, ctypes.POINTER(c_void_p)]),
+    "cadrPropagateTransformsPlanned": (c_int, [c_void_p, c_void_p]),
+    "cadrTransformPlanDestroy": (None, [c_void_p]),
     "cadrComputeWorldSpheres": (c_int, [c_u64, c_u64, c_u64, c_u32, c_u64, c_u64, c_void_p]),
     "cadrMakeView": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_float, c_float,
                              c_void_p, c_void_p]),

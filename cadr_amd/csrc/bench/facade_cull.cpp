@@ -2,7 +2,7 @@
 // roots x 100 x 100 nodes whose leaves are drawn as instances.  Prints one
 // JSON line.
 //
-//   facade_cull <roots> <frames>
+//   facade_cull <roots> <frames> [plan|batch]
 //
 // Scene: 64 Geometries (bounding sphere, 3 LODs as 3 PrimitiveSet records),
 // 64 StateSets under one root.  Every leaf gets a random StateSet and
@@ -13,8 +13,10 @@
 //   1. the inner level: world = root x local (one launch; a root's world
 //      matrix is its local one);
 //   2. the leaf level, propagated straight into the Drawables' MatrixLists
-//      (MatrixList::matricesDeviceAddress(), one transform level per list,
-//      batched 1024 lists per launch);
+//      (MatrixList::matricesDeviceAddress(), one transform level per list):
+//      one launch of a cadrTransformPlan over all lists, its units ordered
+//      by parent (`plan`, the default), or cadrPropagateTransformsBatch,
+//      1024 lists per launch (`batch`);
 //   3. Renderer::recordDrawableProcessing (the reference's gather) and
 //      Renderer::recordDrawableCulling for one view (cull items from the
 //      MatrixLists through the handle table, cull + LOD + per-StateSet
@@ -30,6 +32,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <memory>
 #include <random>
@@ -80,12 +83,13 @@ uint64_t devAlloc(size_t bytes)
 
 int main(int argc, char** argv)
 {
-	if (argc < 3) {
-		std::fprintf(stderr, "usage: facade_cull <roots> <frames>\n");
+	if (argc < 3 || (argc > 3 && std::strcmp(argv[3], "plan") != 0 && std::strcmp(argv[3], "batch") != 0)) {
+		std::fprintf(stderr, "usage: facade_cull <roots> <frames> [plan|batch]\n");
 		return 2;
 	}
 	const size_t R = std::strtoull(argv[1], nullptr, 10), F1 = 100, F2 = 100;
 	const int frames = std::atoi(argv[2]);
+	const bool usePlan = argc <= 3 || std::strcmp(argv[3], "plan") == 0;
 	const size_t inner = R * F1, leaves = inner * F2;
 	const int G = 64, K = 64;
 	try {
@@ -170,6 +174,10 @@ int main(int argc, char** argv)
 		check(hipMemcpy(reinterpret_cast<void*>(dLevels), leafLevels.data(), leafLevels.size() * sizeof(cadrTransformLevel),
 		                hipMemcpyHostToDevice),
 		      "copy");
+		cadrTransformPlan* plan = nullptr;
+		if (usePlan)
+			checkRc(cadrTransformPlanCreate(leafLevels.data(), uint32_t(leafLevels.size()), nullptr, &plan),
+			        "cadrTransformPlanCreate");
 		cadrTransformLevel innerLevel{};
 		innerLevel.localMatrices = dInnerL;
 		innerLevel.parentIndex = dInnerP;
@@ -194,10 +202,15 @@ int main(int argc, char** argv)
 			r.executeCopyOperations();
 			check(hipEventRecord(e0, stream), "hipEventRecord");
 			checkRc(cadrPropagateTransforms(&innerLevel, stream), "inner level");
-			for (size_t b = 0; b < leafLevels.size(); b += 1024) {
-				const uint32_t n = uint32_t(std::min<size_t>(1024, leafLevels.size() - b));
-				checkRc(cadrPropagateTransformsBatch(dLevels + b * sizeof(cadrTransformLevel), &leafLevels[b], n, stream),
-				        "leaf levels");
+			if (usePlan) {
+				checkRc(cadrPropagateTransformsPlanned(plan, stream), "leaf levels (plan)");
+			} else {
+				for (size_t b = 0; b < leafLevels.size(); b += 1024) {
+					const uint32_t n = uint32_t(std::min<size_t>(1024, leafLevels.size() - b));
+					checkRc(cadrPropagateTransformsBatch(dLevels + b * sizeof(cadrTransformLevel), &leafLevels[b], n,
+					                                     stream),
+					        "leaf levels");
+				}
 			}
 			r.beginRecording(stream);
 			const size_t n = r.prepareSceneRendering(*ss[0]);
@@ -227,9 +240,11 @@ int main(int argc, char** argv)
 		std::sort(hostMs.begin(), hostMs.end());
 		std::printf("{\"roots\": %zu, \"leaves\": %zu, \"nodes\": %zu, \"drawables\": %zu, \"cull_items\": %zu, "
 		            "\"statesets\": %d, \"frames\": %d, \"gpu_ms\": %.4f, \"host_ms\": %.4f, \"visible\": %llu, "
-		            "\"handle_level\": %u}\n",
+		            "\"handle_level\": %u, \"leaf_propagation\": \"%s\"}\n",
 		            R, leaves, R + inner + leaves, drawables.size(), r.numCullItems(), K, frames, gpuMs[gpuMs.size() / 2],
-		            hostMs[hostMs.size() / 2], (unsigned long long)visible, r.dataStorage().handleLevel());
+		            hostMs[hostMs.size() / 2], (unsigned long long)visible, r.dataStorage().handleLevel(),
+		            usePlan ? "plan" : "batch");
+		cadrTransformPlanDestroy(plan);
 		(void)hipEventDestroy(e0);
 		(void)hipEventDestroy(e1);
 		(void)hipStreamDestroy(stream);

@@ -20,6 +20,7 @@
 #include "internal.h"
 #include "fp32.h"
 #include "handles.h"
+#include "matrix_io.h"
 
 #include <algorithm>
 
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(256) void buildMeshTableKernel(const uint64_t* __re
 		const uint32_t lods = s.lodCount <= 4u ? s.lodCount : 4u;
 		o.lodCount = lods;
 		for (uint32_t k = 0; k < lods; k++) {
-			const uint32_t* rec = reinterpret_cast<const uint32_t*>(ps + s.primitiveSetOffset + 8ull * k);
+			const GlobalConst<uint32_t> rec = globalAt<uint32_t>(ps + s.primitiveSetOffset + 8ull * k);
 			o.indexCount[k] = rec[0];
 			o.firstIndex[k] = rec[1];
 		}
@@ -81,29 +82,30 @@ __global__ __launch_bounds__(256) void cullItemsKernel(CullItemsArgs a)
 	uint64_t i = it;
 	uint32_t j = 0;
 	if (a.itemBase) {
-		i = a.itemDrawable[it];
-		j = i < a.numDrawables ? uint32_t(it - a.itemBase[i]) : 0u;
+		i = globalAt(a.itemDrawable)[it];
+		j = i < a.numDrawables ? uint32_t(it - globalAt(a.itemBase)[i]) : 0u;
 	}
 	float4 s = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
 	uint32_t mesh = ~0u;
 	if (i < a.numDrawables) {
-		const uint4 w1 = a.list[3 * i + 1];  // matrixListHandle, drawableDataHandle
-		const uint4 w2 = a.list[3 * i + 2];  // primitiveSetHandle, primitiveSetOffset, padding
+		const GlobalConst<uint4> list = globalAt(a.list);
+		const uint4 w1 = loadU4(list + (3 * i + 1));  // matrixListHandle, drawableDataHandle
+		const uint4 w2 = loadU4(list + (3 * i + 2));  // primitiveSetHandle, primitiveSetOffset, padding
 		const uint64_t ml = lookupHandle<LEVEL>(a.root, uint64_t(w1.x) | (uint64_t(w1.y) << 32));
-		const uint32_t numMatrices = ml ? *reinterpret_cast<const uint32_t*>(ml) : 0u;
+		const uint32_t numMatrices = ml ? globalAt<uint32_t>(ml)[0] : 0u;
 		mesh = w2.w;
 		if (j < numMatrices && mesh < a.numMeshes) {
-			const cadrMesh& m = a.meshes[mesh];
-			const float4 local = make_float4(m.center[0], m.center[1], m.center[2], m.radius);
-			const float4* matrices = reinterpret_cast<const float4*>(ml + 64);
+			// the mesh's first 16 bytes: local centre and radius
+			const float4 local = loadF4(globalAt<float4>(reinterpret_cast<uint64_t>(a.meshes + mesh)));
+			const GlobalConst<float4> matrices = globalAt<float4>(ml + 64);
 			float4 M[4];
 #pragma unroll
-			for (int c = 0; c < 4; c++) M[c] = matrices[4ull * j + c];
+			for (int c = 0; c < 4; c++) M[c] = loadF4(matrices + (4ull * j + c));
 			s = sphereTransform(M, local);
 		}
 	}
-	a.spheres[it] = s;
-	a.meshIndex[it] = mesh;
+	storeF4(globalMutAt(a.spheres) + it, s);
+	globalMutAt(a.meshIndex)[it] = mesh;
 }
 
 // `split` workgroups per (view, StateSet) segment, as compactPointersKernel.

@@ -44,11 +44,11 @@ __global__ __launch_bounds__(256) void processDrawablesKernel(const uint64_t* __
 	const uint64_t ml = lookupHandle<LEVEL>(handleTable, matrixListH);
 	uint32_t psCount = 0, psFirst = 0, numMatrices = 0;
 	if (psBase) {
-		const uint32_t* ps = reinterpret_cast<const uint32_t*>(psBase + primitiveSetOffset);
+		const GlobalConst<uint32_t> ps = globalAt<uint32_t>(psBase + primitiveSetOffset);
 		psCount = ps[0];
 		psFirst = ps[1];
 	}
-	if (ml) numMatrices = *reinterpret_cast<const uint32_t*>(ml);
+	if (ml) numMatrices = globalAt<uint32_t>(ml)[0];
 
 	// VkDrawIndirectCommand {vertexCount, instanceCount, firstVertex, firstInstance}
 	indirectData[i] = make_uint4(psCount, numMatrices, psFirst, 0u);

@@ -17,6 +17,7 @@
 // instruction moves one contiguous KiB.
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "internal.h"
 #include "fp32.h"
@@ -75,6 +76,43 @@ __global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict_
 	}
 }
 
+// 64 nodes of one level from `first` on, one per lane of the calling wave,
+// computed as propagateKernel computes them; the root/sphere choice is the
+// level's (uniform in the wave).
+// The descriptor's addresses are used as global (address space 1) pointers:
+// generic ones would compile to FLAT accesses, which count on the LDS counter
+// too, so every wait for the LDS staging would also wait for the unit's
+// matrix loads and stores in flight.
+__device__ __forceinline__ void propagateUnit(const cadrTransformLevel& L, uint64_t first, float4* stage,
+                                              uint32_t lane)
+{
+	typedef const __attribute__((address_space(1))) uint32_t GlobalU32;
+	const uint64_t count = L.count;
+	const uint64_t i = first + lane;
+	const bool live = i < count;
+	float4 M[4], W[4];
+	loadMatrices(reinterpret_cast<const GlobalF4*>(L.localMatrices), first, count, stage, lane, M);
+	const uint32_t p = (L.parentWorld && live) ? reinterpret_cast<GlobalU32*>(L.parentIndex)[i] : ~0u;
+	if (p < L.parentCount) {
+		const GlobalF4* pw = reinterpret_cast<const GlobalF4*>(L.parentWorld);
+		float4 P[4];
+#pragma unroll
+		for (int c = 0; c < 4; c++) P[c] = loadF4(pw + (4 * uint64_t(p) + c));
+		mat4Mul(P, M, W);
+	} else {
+#pragma unroll
+		for (int c = 0; c < 4; c++) W[c] = M[c];
+	}
+	storeMatrices(reinterpret_cast<GlobalF4*>(L.worldMatrices), first, count, stage, lane, W);
+	if (L.spheres && live) {
+		const uint32_t m = reinterpret_cast<GlobalU32*>(L.meshIndex)[i];
+		float4 sphere = make_float4(0.f, 0.f, 0.f, -__builtin_inff());  // BoundingSphere::empty()
+		if (m < L.numMeshes)  // the mesh's first 16 bytes: local centre and radius
+			sphere = sphereTransform(W, loadF4(reinterpret_cast<const GlobalF4*>(L.meshes) + 4 * uint64_t(m)));
+		storeF4(reinterpret_cast<GlobalF4*>(L.spheres) + i, sphere);
+	}
+}
+
 // Many levels (of different trees) in one launch: the levels' nodes are cut
 // into units of 64 (one wave each), numbered level after level; the grid's
 // waves stride over the units.  Each unit is computed exactly as
@@ -88,7 +126,7 @@ __global__ __launch_bounds__(256) void propagateBatchKernel(const cadrTransformL
 {
 	__shared__ float4 stage[4][256];
 	__shared__ uint64_t firstUnit[kMaxBatchLevels + 1];
-	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	// firstUnit[l] = units of levels 0..l-1: loaded in parallel, then an
 	// in-place Hillis-Steele scan over the (at most 1024) levels
 	for (uint32_t i = threadIdx.x; i < numLevels; i += 256u) firstUnit[i + 1] = (levels[i].count + 63u) / 64u;
@@ -122,35 +160,13 @@ __global__ __launch_bounds__(256) void propagateBatchKernel(const cadrTransformL
 				const uint32_t mid = (lo + hi + 1) / 2;
 				if (firstUnit[mid] <= unit) lo = mid; else hi = mid - 1;
 			}
-			l = lo;
+			l = __builtin_amdgcn_readfirstlane(lo);  // the same in every lane
 		}
 		if (l != cached) {  // the descriptor stays in scalar registers while the level lasts
 			L = levels[l];
 			cached = l;
 		}
-		const uint64_t count = L.count;
-		const uint64_t first = (unit - firstUnit[l]) * 64u;
-		const uint64_t i = first + lane;
-		const bool live = i < count;
-		const float4* local = reinterpret_cast<const float4*>(L.localMatrices);
-		float4 M[4], W[4];
-		loadMatrices(local, first, count, stage[wave], lane, M);
-		const uint32_t p = (L.parentWorld && live) ? reinterpret_cast<const uint32_t*>(L.parentIndex)[i] : ~0u;
-		if (p < L.parentCount) {
-			const float4* pw = reinterpret_cast<const float4*>(L.parentWorld);
-			float4 P[4];
-#pragma unroll
-			for (int c = 0; c < 4; c++) P[c] = pw[4 * uint64_t(p) + c];
-			mat4Mul(P, M, W);
-		} else {
-#pragma unroll
-			for (int c = 0; c < 4; c++) W[c] = M[c];
-		}
-		storeMatrices(reinterpret_cast<float4*>(L.worldMatrices), first, count, stage[wave], lane, W);
-		if (L.spheres && live)
-			reinterpret_cast<float4*>(L.spheres)[i] =
-			    nodeSphere(W, reinterpret_cast<const cadrMesh*>(L.meshes), L.numMeshes,
-			               reinterpret_cast<const uint32_t*>(L.meshIndex)[i]);
+		propagateUnit(L, (unit - firstUnit[l]) * 64u, stage[wave], lane);
 	}
 }
 
@@ -167,7 +183,53 @@ __global__ __launch_bounds__(256) void worldSpheresKernel(const float4* __restri
 	spheres[i] = nodeSphere(W, meshes, numMeshes, meshIndex[i]);
 }
 
+// A transform plan's launch: its units (level, first node) in the plan's
+// order, the grid's waves striding over them, so the waves in flight at any
+// time hold consecutive units of that order.
+__global__ __launch_bounds__(256) void propagatePlanKernel(const cadrTransformLevel* __restrict__ levels,
+                                                           const uint2* __restrict__ units, uint64_t numUnits)
+{
+	__shared__ float4 stage[4][256];
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const uint64_t waves = uint64_t(gridDim.x) * 4u;
+	uint64_t u = uint64_t(blockIdx.x) * 4u + wave;
+	if (u >= numUnits) return;
+	// the next unit's record and level descriptor are loaded while the
+	// current unit is computed
+	uint2 unit = units[u];
+	cadrTransformLevel L = levels[unit.x];
+	for (; u < numUnits; u += waves) {
+		const uint64_t un = u + waves;
+		const uint2 next = un < numUnits ? units[un] : unit;
+		const cadrTransformLevel Ln = levels[next.x];
+		propagateUnit(L, uint64_t(unit.y), stage[wave], lane);
+		unit = next;
+		L = Ln;
+	}
+}
+
+// The sort key of each unit of a plan: the parent index of its first node
+// (0 for a root level, which reads no parent).
+__global__ __launch_bounds__(256) void planKeysKernel(const cadrTransformLevel* __restrict__ levels,
+                                                      const uint2* __restrict__ units, uint64_t numUnits,
+                                                      uint32_t* __restrict__ keys)
+{
+	const uint64_t u = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+	if (u >= numUnits) return;
+	const uint2 unit = units[u];
+	const cadrTransformLevel& L = levels[unit.x];
+	keys[u] = L.parentWorld ? reinterpret_cast<const uint32_t*>(L.parentIndex)[unit.y] : 0u;
+}
+
 }  // namespace cadr
+
+struct cadrTransformPlan {
+	int device = 0;
+	cadrTransformLevel* levels = nullptr;  // device copy of the descriptors
+	uint2* units = nullptr;                // (level, first node), in launch order
+	uint64_t numUnits = 0;
+};
 
 using namespace cadr;
 
@@ -251,4 +313,88 @@ CADR_API int cadrComputeWorldSpheres(uint64_t worldMatrices, uint64_t meshIndex,
 	                   reinterpret_cast<const cadrMesh*>(meshes), numMeshes, count, reinterpret_cast<float4*>(spheres));
 	CADR_CHECK_HIP(hipGetLastError(), "worldSpheresKernel launch");
 	return ok();
+}
+
+CADR_API int cadrTransformPlanCreate(const cadrTransformLevel* hostLevels, uint32_t numLevels, cadrStream stream,
+                                     cadrTransformPlan** plan)
+{
+	CADR_REQUIRE(plan, "cadrTransformPlanCreate: plan is NULL");
+	*plan = nullptr;
+	CADR_REQUIRE(numLevels == 0 || hostLevels, "cadrTransformPlanCreate: hostLevels is NULL");
+	std::vector<uint2> units;
+	for (uint32_t l = 0; l < numLevels; l++) {
+		if (hostLevels[l].count == 0) continue;
+		const int rc = checkLevel(&hostLevels[l], "cadrTransformPlanCreate");
+		if (rc != CADR_SUCCESS) return rc;
+		CADR_REQUIRE(hostLevels[l].count <= 0xffffffffull, "cadrTransformPlanCreate: a level holds 2^32 nodes or more");
+		for (uint64_t f = 0; f < hostLevels[l].count; f += 64) units.push_back(make_uint2(l, uint32_t(f)));
+	}
+	auto* p = new cadrTransformPlan;
+	CADR_CHECK_HIP(hipGetDevice(&p->device), "hipGetDevice");
+	p->numUnits = units.size();
+	hipStream_t s = hipStream_t(stream);
+	uint32_t* dKeys = nullptr;
+	auto fail = [&](hipError_t e, const char* what) {
+		(void)hipFree(dKeys);
+		cadrTransformPlanDestroy(p);
+		return setHipError(e, what);
+	};
+	hipError_t e = hipMalloc(&p->levels, std::max<size_t>(numLevels, 1) * sizeof(cadrTransformLevel));
+	if (e != hipSuccess) return fail(e, "hipMalloc(plan levels)");
+	e = hipMalloc(&p->units, std::max<size_t>(units.size(), 1) * sizeof(uint2));
+	if (e != hipSuccess) return fail(e, "hipMalloc(plan units)");
+	if (!units.empty()) {
+		e = hipMalloc(&dKeys, units.size() * sizeof(uint32_t));
+		if (e != hipSuccess) return fail(e, "hipMalloc(plan keys)");
+		e = hipMemcpyAsync(p->levels, hostLevels, numLevels * sizeof(cadrTransformLevel), hipMemcpyHostToDevice, s);
+		if (e == hipSuccess) e = hipMemcpyAsync(p->units, units.data(), units.size() * sizeof(uint2), hipMemcpyHostToDevice, s);
+		if (e != hipSuccess) return fail(e, "hipMemcpyAsync(plan)");
+		hipLaunchKernelGGL(planKeysKernel, dim3(uint32_t((units.size() + 255) / 256)), dim3(256), 0, s, p->levels,
+		                   p->units, uint64_t(units.size()), dKeys);
+		e = hipGetLastError();
+		if (e != hipSuccess) return fail(e, "planKeysKernel launch");
+		std::vector<uint32_t> keys(units.size());
+		e = hipMemcpyAsync(keys.data(), dKeys, keys.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+		if (e == hipSuccess) e = hipStreamSynchronize(s);
+		if (e != hipSuccess) return fail(e, "hipMemcpyAsync(plan keys)");
+		// order: by the parent array, then by the parent of the unit's first
+		// node; equal keys keep their level-by-level order
+		std::vector<uint32_t> order(units.size());
+		for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+			const uint64_t pa = hostLevels[units[a].x].parentWorld, pb = hostLevels[units[b].x].parentWorld;
+			if (pa != pb) return pa < pb;
+			return keys[a] < keys[b];
+		});
+		std::vector<uint2> sorted(units.size());
+		for (size_t i = 0; i < order.size(); i++) sorted[i] = units[order[i]];
+		e = hipMemcpy(p->units, sorted.data(), sorted.size() * sizeof(uint2), hipMemcpyHostToDevice);
+		if (e != hipSuccess) return fail(e, "hipMemcpy(plan units)");
+		(void)hipFree(dKeys);
+		dKeys = nullptr;
+	}
+	*plan = p;
+	return ok();
+}
+
+CADR_API int cadrPropagateTransformsPlanned(const cadrTransformPlan* plan, cadrStream stream)
+{
+	CADR_REQUIRE(plan, "cadrPropagateTransformsPlanned: plan is NULL");
+	if (plan->numUnits == 0) return ok();
+	int dev = 0;
+	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
+	CADR_REQUIRE(dev == plan->device, "cadrPropagateTransformsPlanned: the plan belongs to another device");
+	const uint64_t blocks = std::min<uint64_t>((plan->numUnits + 3) / 4, uint64_t(computeUnitCount()) * 16u);
+	hipLaunchKernelGGL(propagatePlanKernel, dim3(uint32_t(blocks)), dim3(256), 0, hipStream_t(stream), plan->levels,
+	                   plan->units, plan->numUnits);
+	CADR_CHECK_HIP(hipGetLastError(), "propagatePlanKernel launch");
+	return ok();
+}
+
+CADR_API void cadrTransformPlanDestroy(cadrTransformPlan* plan)
+{
+	if (!plan) return;
+	(void)hipFree(plan->levels);
+	(void)hipFree(plan->units);
+	delete plan;
 }

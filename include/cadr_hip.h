@@ -263,6 +263,24 @@ int cadrPropagateTransforms(const cadrTransformLevel* level, cadrStream stream);
 int cadrPropagateTransformsBatch(uint64_t deviceLevels, const cadrTransformLevel* hostLevels, uint32_t numLevels,
                                  cadrStream stream);
 
+/* A fixed batch of independent levels (the rules of
+ * cadrPropagateTransformsBatch, any number of levels) whose 64-node work
+ * units are ordered by the parent of their first node: the units in flight
+ * at any time read nearby parent matrices, which then stay in the L2 where
+ * many levels share one parent array and each level's nodes are ordered by
+ * parent (the instances of a mesh gathered from a scene graph, each set of
+ * instances a level).  Results are those of cadrPropagateTransformsBatch;
+ * only the order of the work differs.  Create reads the levels' parent
+ * indices (synchronising the stream); the plan keeps copies of the
+ * descriptors, so the arrays they name must stay in place while it is used.
+ * Replaces the per-parent recursion of gltfReader/main.cpp:1538-1576 for
+ * instanced MatrixLists; no reference counterpart of its own. */
+typedef struct cadrTransformPlan cadrTransformPlan;
+int cadrTransformPlanCreate(const cadrTransformLevel* hostLevels, uint32_t numLevels, cadrStream stream,
+                            cadrTransformPlan** plan);
+int cadrPropagateTransformsPlanned(const cadrTransformPlan* plan, cadrStream stream);
+void cadrTransformPlanDestroy(cadrTransformPlan* plan);
+
 /* World spheres of drawables whose world matrices are already known (flat
  * scenes): spheres[i] = worldMatrices[i] * meshes[meshIndex[i]].sphere. */
 int cadrComputeWorldSpheres(uint64_t worldMatrices, uint64_t meshIndex, uint64_t meshes,
