@@ -485,12 +485,17 @@ static void gpuFrame(int handleLevel, size_t numDrawables, uint32_t seed)
 	r.setCollectFrameInfo(true);
 	hipStream_t stream;
 	if (hipStreamCreate(&stream) != hipSuccess) throw std::runtime_error("hipStreamCreate");
+	hipEvent_t outerBegin, outerEnd;
+	if (hipEventCreate(&outerBegin) != hipSuccess || hipEventCreate(&outerEnd) != hipSuccess)
+		throw std::runtime_error("hipEventCreate");
 	for (int frame = 0; frame < 2; frame++) {
 		r.beginFrame();
 		r.executeCopyOperations();
 		r.beginRecording(stream);
 		const size_t n = r.prepareSceneRendering(*sc.stateSets[0]);
+		if (hipEventRecord(outerBegin, stream) != hipSuccess) throw std::runtime_error("hipEventRecord");
 		r.recordDrawableProcessing(stream, n);
+		if (hipEventRecord(outerEnd, stream) != hipSuccess) throw std::runtime_error("hipEventRecord");
 		std::vector<DrawRange> ranges;
 		r.recordSceneRendering(stream, *sc.stateSets[0], ranges);
 		r.endRecording(stream);
@@ -529,10 +534,60 @@ static void gpuFrame(int handleLevel, size_t numDrawables, uint32_t seed)
 		CHECK(fi.frameNumber == size_t(frame) && fi.gpuEndExecution >= fi.gpuAfterDrawableProcessingAndBeforeRendering &&
 		          fi.gpuAfterDrawableProcessingAndBeforeRendering >= fi.gpuAfterTransfersAndBeforeDrawableProcessing,
 		      "frame info timestamps are ordered");
+		CHECK(fi.cpuBeginFrame <= fi.cpuPrepareRecordingBegin && fi.cpuPrepareRecordingBegin <= fi.cpuPrepareRecordingEnd &&
+		          fi.cpuPrepareRecordingEnd <= fi.cpuRecordStateSetsBegin &&
+		          fi.cpuRecordStateSetsBegin <= fi.cpuRecordStateSetsEnd && fi.cpuRecordStateSetsEnd <= fi.cpuEndFrame &&
+		          fi.cpuBeginFrame > 0,
+		      "frame info CPU timestamps are taken and ordered");
+		// the drawable-processing interval (Renderer.cpp:659-702: ts2 - ts1)
+		// lies inside the caller's own events around recordDrawableProcessing
+		float outerMs = 0.f;
+		if (hipEventElapsedTime(&outerMs, outerBegin, outerEnd) != hipSuccess) throw std::runtime_error("elapsed");
+		const uint64_t processing =
+		    fi.gpuAfterDrawableProcessingAndBeforeRendering - fi.gpuAfterTransfersAndBeforeDrawableProcessing;
+		CHECK(processing > 0 && double(processing) <= double(outerMs) * 1e6 + 20000.0,
+		      "drawable processing " + std::to_string(processing) + " ns inside the caller's " +
+		          std::to_string(outerMs) + " ms");
 		// second frame: move one MatrixList (new address under the same handle)
 		float m[32] = {};
 		sc.matrixLists[0]->setMatrices(m, 2);
 	}
+	(void)hipEventDestroy(outerBegin);
+	(void)hipEventDestroy(outerEnd);
+	(void)hipStreamDestroy(stream);
+}
+
+// A frame with no drawables: recordDrawableProcessing dispatches nothing and
+// records the two timestamps back to back (Renderer.cpp:600-620), so the
+// drawable-processing interval is (near) zero and the frame's timestamps stay
+// ordered.
+static void gpuEmptyFrame()
+{
+	std::printf("gpu empty frame\n");
+	std::fflush(stdout);
+	Renderer r(0, false);
+	StateSet root(r);
+	r.setCollectFrameInfo(true);
+	hipStream_t stream;
+	if (hipStreamCreate(&stream) != hipSuccess) throw std::runtime_error("hipStreamCreate");
+	r.beginFrame();
+	r.executeCopyOperations();
+	r.beginRecording(stream);
+	const size_t n = r.prepareSceneRendering(root);
+	CHECK(n == 0, "an empty scene counts no drawables");
+	r.recordDrawableProcessing(stream, n);
+	std::vector<DrawRange> ranges;
+	r.recordSceneRendering(stream, root, ranges);
+	r.endRecording(stream);
+	r.endFrame();
+	if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("hipStreamSynchronize");
+	const FrameInfo& fi = r.getFrameInfo();
+	const uint64_t processing =
+	    fi.gpuAfterDrawableProcessingAndBeforeRendering - fi.gpuAfterTransfersAndBeforeDrawableProcessing;
+	CHECK(fi.gpuAfterDrawableProcessingAndBeforeRendering >= fi.gpuAfterTransfersAndBeforeDrawableProcessing &&
+	          fi.gpuEndExecution >= fi.gpuAfterDrawableProcessingAndBeforeRendering && processing < 200000,
+	      "empty frame: processing interval " + std::to_string(processing) + " ns");
+	CHECK(ranges.empty() || ranges[0].count == 0, "an empty scene draws nothing");
 	(void)hipStreamDestroy(stream);
 }
 
@@ -956,6 +1011,7 @@ int main(int argc, char** argv)
 			testParentChildLists();
 			testStateSetOrder();
 		} else if (mode == "gpu") {
+			gpuEmptyFrame();
 			gpuFrame(1, 500, 1);
 			gpuFrame(2, 20000, 2);
 			gpuFrame(3, 20000, 3);
