@@ -83,6 +83,7 @@ __global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict_
 // generic ones would compile to FLAT accesses, which count on the LDS counter
 // too, so every wait for the LDS staging would also wait for the unit's
 // matrix loads and stores in flight.
+template <bool SPHERES>
 __device__ __forceinline__ void propagateUnit(const cadrTransformLevel& L, uint64_t first, float4* stage,
                                               uint32_t lane)
 {
@@ -104,7 +105,7 @@ __device__ __forceinline__ void propagateUnit(const cadrTransformLevel& L, uint6
 		for (int c = 0; c < 4; c++) W[c] = M[c];
 	}
 	storeMatrices(reinterpret_cast<GlobalF4*>(L.worldMatrices), first, count, stage, lane, W);
-	if (L.spheres && live) {
+	if (SPHERES && L.spheres && live) {
 		const uint32_t m = reinterpret_cast<GlobalU32*>(L.meshIndex)[i];
 		float4 sphere = make_float4(0.f, 0.f, 0.f, -__builtin_inff());  // BoundingSphere::empty()
 		if (m < L.numMeshes)  // the mesh's first 16 bytes: local centre and radius
@@ -121,7 +122,8 @@ __device__ __forceinline__ void propagateUnit(const cadrTransformLevel& L, uint6
 // other; the caller orders dependent levels into successive launches.
 constexpr uint32_t kMaxBatchLevels = 1024;
 
-__global__ __launch_bounds__(256) void propagateBatchKernel(const cadrTransformLevel* __restrict__ levels,
+template <bool SPHERES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void propagateBatchKernel(const cadrTransformLevel* __restrict__ levels,
                                                             uint32_t numLevels)
 {
 	__shared__ float4 stage[4][256];
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void propagateBatchKernel(const cadrTransformL
 			L = levels[l];
 			cached = l;
 		}
-		propagateUnit(L, (unit - firstUnit[l]) * 64u, stage[wave], lane);
+		propagateUnit<SPHERES>(L, (unit - firstUnit[l]) * 64u, stage[wave], lane);
 	}
 }
 
@@ -183,30 +185,21 @@ __global__ __launch_bounds__(256) void worldSpheresKernel(const float4* __restri
 	spheres[i] = nodeSphere(W, meshes, numMeshes, meshIndex[i]);
 }
 
-// A transform plan's launch: its units (level, first node) in the plan's
-// order, the grid's waves striding over them, so the waves in flight at any
-// time hold consecutive units of that order.
-__global__ __launch_bounds__(256) void propagatePlanKernel(const cadrTransformLevel* __restrict__ levels,
+// A transform plan's launch: one unit (level, first node) per wave, in the
+// plan's order; workgroups are dispatched in index order, so the waves in
+// flight at any time hold consecutive units of that order.
+template <bool SPHERES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void propagatePlanKernel(const cadrTransformLevel* __restrict__ levels,
                                                            const uint2* __restrict__ units, uint64_t numUnits)
 {
 	__shared__ float4 stage[4][256];
 	const uint32_t lane = threadIdx.x & 63u;
 	const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	const uint64_t waves = uint64_t(gridDim.x) * 4u;
-	uint64_t u = uint64_t(blockIdx.x) * 4u + wave;
-	if (u >= numUnits) return;
-	// the next unit's record and level descriptor are loaded while the
-	// current unit is computed
-	uint2 unit = units[u];
-	cadrTransformLevel L = levels[unit.x];
-	for (; u < numUnits; u += waves) {
-		const uint64_t un = u + waves;
-		const uint2 next = un < numUnits ? units[un] : unit;
-		const cadrTransformLevel Ln = levels[next.x];
-		propagateUnit(L, uint64_t(unit.y), stage[wave], lane);
-		unit = next;
-		L = Ln;
-	}
+	const uint64_t u = uint64_t(blockIdx.x) * 4u + wave;
+	if (u >= numUnits) return;  // whole wave past the end (waves are independent)
+	const uint2 unit = units[u];
+	const cadrTransformLevel L = levels[unit.x];
+	propagateUnit<SPHERES>(L, uint64_t(unit.y), stage[wave], lane);
 }
 
 // The sort key of each unit of a plan: the parent index of its first node
@@ -229,6 +222,7 @@ struct cadrTransformPlan {
 	cadrTransformLevel* levels = nullptr;  // device copy of the descriptors
 	uint2* units = nullptr;                // (level, first node), in launch order
 	uint64_t numUnits = 0;
+	bool spheres = false;                  // some level writes world spheres
 };
 
 using namespace cadr;
@@ -285,16 +279,20 @@ CADR_API int cadrPropagateTransformsBatch(uint64_t deviceLevels, const cadrTrans
 	CADR_REQUIRE(numLevels <= kMaxBatchLevels, "cadrPropagateTransformsBatch: more than 1024 levels");
 	CADR_REQUIRE(deviceLevels % 8 == 0, "cadrPropagateTransformsBatch: misaligned level array");
 	uint64_t units = 0;
+	bool spheres = false;
 	for (uint32_t l = 0; l < numLevels; l++) {
 		if (hostLevels[l].count == 0) continue;
 		const int rc = checkLevel(&hostLevels[l], "cadrPropagateTransformsBatch");
 		if (rc != CADR_SUCCESS) return rc;
 		units += (hostLevels[l].count + 63u) / 64u;
+		spheres = spheres || hostLevels[l].spheres != 0;
 	}
 	if (units == 0) return ok();
+	// without world spheres the kernel holds fewer registers: 8 waves per SIMD instead of 5
 	const uint64_t blocks = std::min<uint64_t>((units + 3) / 4, uint64_t(computeUnitCount()) * 16u);
-	hipLaunchKernelGGL(propagateBatchKernel, dim3(uint32_t(blocks)), dim3(256), 0, hipStream_t(stream),
-	                   reinterpret_cast<const cadrTransformLevel*>(deviceLevels), numLevels);
+	hipLaunchKernelGGL(spheres ? propagateBatchKernel<true> : propagateBatchKernel<false>, dim3(uint32_t(blocks)),
+	                   dim3(256), 0, hipStream_t(stream), reinterpret_cast<const cadrTransformLevel*>(deviceLevels),
+	                   numLevels);
 	CADR_CHECK_HIP(hipGetLastError(), "propagateBatchKernel launch");
 	return ok();
 }
@@ -330,6 +328,7 @@ CADR_API int cadrTransformPlanCreate(const cadrTransformLevel* hostLevels, uint3
 		for (uint64_t f = 0; f < hostLevels[l].count; f += 64) units.push_back(make_uint2(l, uint32_t(f)));
 	}
 	auto* p = new cadrTransformPlan;
+	for (uint32_t l = 0; l < numLevels; l++) p->spheres = p->spheres || (hostLevels[l].count && hostLevels[l].spheres);
 	CADR_CHECK_HIP(hipGetDevice(&p->device), "hipGetDevice");
 	p->numUnits = units.size();
 	hipStream_t s = hipStream_t(stream);
@@ -384,9 +383,10 @@ CADR_API int cadrPropagateTransformsPlanned(const cadrTransformPlan* plan, cadrS
 	int dev = 0;
 	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
 	CADR_REQUIRE(dev == plan->device, "cadrPropagateTransformsPlanned: the plan belongs to another device");
-	const uint64_t blocks = std::min<uint64_t>((plan->numUnits + 3) / 4, uint64_t(computeUnitCount()) * 16u);
-	hipLaunchKernelGGL(propagatePlanKernel, dim3(uint32_t(blocks)), dim3(256), 0, hipStream_t(stream), plan->levels,
-	                   plan->units, plan->numUnits);
+	const uint64_t blocks = (plan->numUnits + 3) / 4;
+	CADR_REQUIRE(blocks < (1ull << 31), "cadrPropagateTransformsPlanned: plan too large");
+	hipLaunchKernelGGL(plan->spheres ? propagatePlanKernel<true> : propagatePlanKernel<false>, dim3(uint32_t(blocks)),
+	                   dim3(256), 0, hipStream_t(stream), plan->levels, plan->units, plan->numUnits);
 	CADR_CHECK_HIP(hipGetLastError(), "propagatePlanKernel launch");
 	return ok();
 }
