@@ -37,7 +37,7 @@ __device__ __forceinline__ float4 nodeSphere(const float4 W[4], const cadrMesh* 
 }
 
 template <bool ROOT, bool SPHERES>
-__global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict__ local,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void propagateKernel(const float4* __restrict__ local,
                                                        const uint32_t* __restrict__ parentIndex,
                                                        const float4* __restrict__ parentWorld,
                                                        uint32_t parentCount, float4* __restrict__ world,
@@ -52,23 +52,34 @@ __global__ __launch_bounds__(256) void propagateKernel(const float4* __restrict_
 	if (first >= count) return;  // whole wave past the end (waves are independent)
 	const uint64_t i = first + lane;
 	const bool live = i < count;
+	if constexpr (ROOT && !SPHERES) {
+		// world = local: a straight copy, each instruction one contiguous KiB
+		// (through registers and LDS the compiler kept the matrices on the
+		// scratch stack)
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			const uint32_t q = uint32_t(k) * 64u + lane;
+			if (first + q / 4u < count) storeF4Stream(world + (4 * first + q), loadF4Stream(local + (4 * first + q)));
+		}
+		return;
+	}
 	float4 L[4];
 	loadMatrices(local, first, count, stage[wave], lane, L);
-	float4 W[4];
 	if constexpr (ROOT) {
+		storeMatrices(world, first, count, stage[wave], lane, L);
+		if (live) spheres[i] = nodeSphere(L, meshes, numMeshes, meshIndex[i]);
+		return;
+	}
+	float4 W[4];
+	const uint32_t p = live ? parentIndex[i] : ~0u;
+	if (p < parentCount) {
+		float4 P[4];
+#pragma unroll
+		for (int c = 0; c < 4; c++) P[c] = parentWorld[4 * uint64_t(p) + c];
+		mat4Mul(P, L, W);
+	} else {
 #pragma unroll
 		for (int c = 0; c < 4; c++) W[c] = L[c];
-	} else {
-		const uint32_t p = live ? parentIndex[i] : ~0u;
-		if (p < parentCount) {
-			float4 P[4];
-#pragma unroll
-			for (int c = 0; c < 4; c++) P[c] = parentWorld[4 * uint64_t(p) + c];
-			mat4Mul(P, L, W);
-		} else {
-#pragma unroll
-			for (int c = 0; c < 4; c++) W[c] = L[c];
-		}
 	}
 	storeMatrices(world, first, count, stage[wave], lane, W);
 	if constexpr (SPHERES) {
