@@ -72,6 +72,7 @@ TILE_LAST = 0x40000000
 TILE_STATESET_MASK = 0x3FFFFFFF
 CULL_NO_TILE_SPHERES = 1
 CULL_WHOLE_TILES = 2
+CULL_PATH_VIEW, CULL_PATH_BATCH, CULL_PATH_FUSED_VIEW, CULL_PATH_FUSED_BATCH = 1, 2, 4, 8
 
 SUCCESS = 0
 ERROR_INVALID_VALUE = 1
@@ -178,6 +179,7 @@ SIGNATURES = {
     "cadrCullBatchNoAlloc": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_u32, c_void_p]),
     "cadrCullView": (c_int, [ctypes.POINTER(CullConfig), c_u64, c_void_p, c_void_p, c_void_p]),
     "cadrCullTileSize": (c_u32, [ctypes.POINTER(CullConfig)]),
+    "cadrCullConfigPaths": (c_u32, [ctypes.POINTER(CullConfig)]),
     "cadrPropagateCullView": (c_int, [c_void_p, ctypes.POINTER(CullConfig), c_u64, c_void_p, c_void_p, c_void_p]),
     "cadrCompactDrawablePointers": (c_int, [c_u64, c_u64, c_u64, c_u32, c_u32, c_u32, c_u64, c_u64, c_u64,
                                             c_void_p]),

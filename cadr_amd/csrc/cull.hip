@@ -818,7 +818,7 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 	const uint32_t lane = laneId();
 	const uint32_t wave = tid / 64;
 	const uint32_t epoch = __hip_atomic_load(&P.ctl[kCtlEpoch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	uint32_t job = 0;
+	uint32_t job = 0, stagedJob = ~0u;
 
 	for (;;) {
 		if (tid == 0) sh.ticket = atomicAdd(&P.ctl[kCtlTicket], 1u);
@@ -852,7 +852,14 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 		}
 		const bool storeLeaf = gBegin == 0;
 		const TileDesc T = loadTile(J, t, S::TILE);
-		const bool ldsMeshes = stageMeshes<BLOCK>(sh.meshes, J, tid);
+		// the job's mesh table is staged once per run of tickets of one job
+		// (every wave left the previous tile's last barrier, so no wave still
+		// reads the old table)
+		bool ldsMeshes = J.numMeshes <= kLdsMeshes;
+		if (job != stagedJob) {
+			ldsMeshes = stageMeshes<BLOCK>(sh.meshes, J, tid);
+			stagedJob = job;
+		}
 
 		const MeshTable meshes{sh.meshes, globalPtr<cadrMesh>(J.meshes), ldsMeshes};
 
@@ -1283,6 +1290,14 @@ CADR_API uint32_t cadrCullTileSize(const cadrCullConfig* config)
 {
 	const KernelEntry* k = findKernel(config);
 	return k ? k->block * k->items : 0u;
+}
+
+CADR_API uint32_t cadrCullConfigPaths(const cadrCullConfig* config)
+{
+	const KernelEntry* k = findKernel(config);
+	if (!k) return 0u;
+	return (k->view || k->batch ? CADR_CULL_PATH_VIEW : 0u) | (k->batch ? CADR_CULL_PATH_BATCH : 0u) |
+	       (k->fused ? CADR_CULL_PATH_FUSED_VIEW : 0u) | (k->fusedBatch ? CADR_CULL_PATH_FUSED_BATCH : 0u);
 }
 
 CADR_API int cadrBuildCullTiles(const uint32_t* stateSetCounts, uint32_t numStateSets, uint32_t tileSize,

@@ -379,7 +379,8 @@ void Renderer::forgetGeometry(const Geometry* g) noexcept
 
 void Renderer::setCullConfig(const cadrCullConfig& singleView, const cadrCullConfig& batched)
 {
-	if (cadrCullTileSize(&singleView) == 0 || cadrCullTileSize(&batched) == 0)
+	if (!(cadrCullConfigPaths(&singleView) & CADR_CULL_PATH_VIEW) ||
+	    !(cadrCullConfigPaths(&batched) & CADR_CULL_PATH_BATCH))
 		throw LogicError("CadR::Renderer::setCullConfig(): unsupported blockSize / itemsPerThread");
 	_cullSingle = singleView;
 	_cullBatched = batched;

@@ -473,6 +473,15 @@ int cadrPropagateCullBatch(uint64_t deviceLeaves, const cadrTransformLevel* host
 /* Kernel tile size for a config (blockSize * itemsPerThread), 0 if unsupported. */
 uint32_t cadrCullTileSize(const cadrCullConfig* config);
 
+/* The entry points that have a kernel of this shape (0: none):
+ * cadrCullView (which runs the batched kernel when the shape has no one-view
+ * kernel), cadrCullBatch, cadrPropagateCullView, cadrPropagateCullBatch. */
+#define CADR_CULL_PATH_VIEW 1u
+#define CADR_CULL_PATH_BATCH 2u
+#define CADR_CULL_PATH_FUSED_VIEW 4u
+#define CADR_CULL_PATH_FUSED_BATCH 8u
+uint32_t cadrCullConfigPaths(const cadrCullConfig* config);
+
 /* Autotuner of the cull launch shape (workgroup size, drawables per thread,
  * hence the scan tile, tile-sphere classification, workgroups per CU).
  * Times every candidate on the caller's own jobs — real launches that write

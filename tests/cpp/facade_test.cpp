@@ -319,6 +319,17 @@ static void testBufferGrowth()
 	ds.resize(10);
 	CHECK(r.prepareSceneRendering(root) == 10 && r.bufferCapacity() == 1200, "buffers never shrink");
 	ds.clear();
+
+	// cull shapes: the one-view shape needs a one-view (or batched) kernel,
+	// the batched shape a batched kernel
+	r.setCullConfig(cadrCullConfig{1024, 8, 0, 0}, cadrCullConfig{256, 8, 0, 0});
+	bool refused = false;
+	try {
+		r.setCullConfig(cadrCullConfig{1024, 8, 0, 0}, cadrCullConfig{1024, 8, 0, 0});
+	} catch (const LogicError&) {
+		refused = true;
+	}
+	CHECK(refused, "a batched cull shape without a batched kernel is refused");
 }
 
 static void testParentChildLists()

@@ -102,6 +102,26 @@ def test_cull_config_tile_sizes():
     assert cull.tile_size(cull.make_config(100, 3)) == 0
 
 
+def test_cull_config_paths():
+    """Which entry points have a kernel of each shape: every one-view shape
+    (256/512/1024 threads), the batched shapes, the fused ones."""
+    lib = N.load()
+
+    def paths(b, i):
+        return lib.cadrCullConfigPaths(ctypes.byref(cull.make_config(b, i)))
+
+    V, B, FV, FB = N.CULL_PATH_VIEW, N.CULL_PATH_BATCH, N.CULL_PATH_FUSED_VIEW, N.CULL_PATH_FUSED_BATCH
+    assert paths(256, 8) == V | B | FV | FB
+    assert paths(512, 4) == V | B | FV | FB
+    assert paths(256, 16) == V | B | FV
+    assert paths(1024, 8) == V  # one view only
+    assert paths(1024, 2) == V | FV
+    assert paths(128, 8) == V | B  # the one-view entry runs the batched kernel
+    assert paths(100, 3) == 0
+    for b, i in [(256, 2), (256, 4), (512, 8), (512, 16), (1024, 1), (1024, 4), (1024, 16)]:
+        assert paths(b, i) & V and cull.tile_size(cull.make_config(b, i)) == b * i
+
+
 def test_result_strings():
     lib = N.load()
     assert lib.cadrGetResultString(0) == b"CADR_SUCCESS"
