@@ -103,7 +103,7 @@ def parse():
     p.add_argument("--no-facade", action="store_true", help="skip the facade cull frames")
     p.add_argument("--no-hbm-probe", action="store_true", help="skip the measured copy / read / write bandwidth")
     p.add_argument("--resident-drawables", type=int, default=10_000_000)
-    p.add_argument("--served-rates", default="5000,10000,20000,50000,100000,200000",
+    p.add_argument("--served-rates", default="5000,10000,20000,50000,65000,80000,100000,200000",
                    help="Poisson arrival rates (req/s)")
     p.add_argument("--served-overload-gib", type=float, default=1.0,
                    help="output budget of the overload run (0: skip it)")
