@@ -103,7 +103,7 @@ def parse():
     p.add_argument("--no-facade", action="store_true", help="skip the facade cull frames")
     p.add_argument("--no-hbm-probe", action="store_true", help="skip the measured copy / read / write bandwidth")
     p.add_argument("--resident-drawables", type=int, default=10_000_000)
-    p.add_argument("--served-rates", default="5000,10000,20000,50000,65000,80000,100000,200000",
+    p.add_argument("--served-rates", default="5000,10000,20000,50000,65000,80000,90000,100000,200000",
                    help="Poisson arrival rates (req/s)")
     p.add_argument("--served-overload-gib", type=float, default=1.0,
                    help="output budget of the overload run (0: skip it)")
@@ -775,6 +775,15 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     return res
 
 
+# Launches in flight in the serving loop.  Measured on served128 (256 x 8, no
+# tile spheres, p99 at 50k / 65k / 80k / 100k req/s): 1 in flight 0.92 /
+# 1.36 / 1.75 / 2.59 ms, 2 in flight 0.98 / 1.26 / 1.66 / 2.49 ms, 3 0.85 /
+# 1.32 / 1.82 / 2.81 ms, 4 0.97 / 1.71 / 2.39 / 3.89 ms, 8 1.11 / 2.08 /
+# 3.00 / 5.13 ms: persistent launches running at once share the CUs, so
+# beyond two the extra overlap costs more than it hides.
+SERVED_INFLIGHT = 2
+
+
 def bench_served(args, config, N, cull, scenes, device):
     """served128: 128 resident flat scenes (generated in HBM); Poisson cull
     requests, 80% to the 16 hottest scenes, admitted by output memory and
@@ -798,8 +807,8 @@ def bench_served(args, config, N, cull, scenes, device):
         log(f"served128 autotune (16 scenes x 2 views): {describe(config)}  {tuned['candidates_ms']}")
         for out in outs:
             out.close()
-    srv = Server(block=config.blockSize, items=config.itemsPerThread, flags=config.flags, max_batch_views=256, inflight=4,
-                 budget_bytes=int(args.served_budget_gib * (1 << 30)))
+    srv = Server(block=config.blockSize, items=config.itemsPerThread, flags=config.flags, max_batch_views=256,
+                 inflight=SERVED_INFLIGHT, budget_bytes=int(args.served_budget_gib * (1 << 30)))
     for ds in dss:
         srv.add_scene(ds)
     srv.start()
@@ -822,7 +831,7 @@ def bench_served(args, config, N, cull, scenes, device):
     overload = None
     if args.served_overload_gib > 0:
         small = Server(block=config.blockSize, items=config.itemsPerThread, flags=config.flags, max_batch_views=256,
-                       inflight=4, budget_bytes=int(args.served_overload_gib * (1 << 30)), max_queued=4096)
+                       inflight=SERVED_INFLIGHT, budget_bytes=int(args.served_overload_gib * (1 << 30)), max_queued=4096)
         for ds in dss:
             small.add_scene(ds)
         small.start()
@@ -846,7 +855,7 @@ def bench_served(args, config, N, cull, scenes, device):
             "workload": "served128: 128 flat scenes of round(exp(U[ln 1e4, ln 2e7])) drawables (64 StateSets, 3 LODs), "
                         "Poisson arrivals, 80% of requests to the 16 hottest scenes, views as batch256",
             "drawables_resident": int(sum(sizes)), "output_budget_gib": args.served_budget_gib,
-            "max_batch_views": 256, "inflight_batches": 4,
+            "max_batch_views": 256, "inflight_batches": SERVED_INFLIGHT,
             "config": {"block": config.blockSize, "items_per_thread": config.itemsPerThread, "flags": config.flags,
                        "autotune": tuned},
             "runs": runs}
