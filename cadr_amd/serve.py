@@ -9,7 +9,7 @@ from . import _native as N
 
 
 class Server:
-    def __init__(self, block=256, items=4, max_batch_views=256, inflight=4, budget_bytes=8 << 30,
+    def __init__(self, block=256, items=4, max_batch_views=256, inflight=2, budget_bytes=8 << 30,
                  max_queued=1 << 20, flags=0):
         cfg = N.ServerConfig(N.CullConfig(block, items, 0, flags), max_batch_views, inflight, budget_bytes,
                              max_queued, 0)
