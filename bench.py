@@ -819,10 +819,16 @@ def bench_served(args, config, N, cull, scenes, device):
         st = srv.run_load(views, rate=rate, duration=args.served_seconds, seed=int(rate), hot=16, hot_fraction=0.8)
         st.pop("samples")
         st["offered_rate"] = rate
+        # algorithmic bytes of the run's launches: each job's scene arrays
+        # once, plus the commands written, over the run's span
+        alg = st["sceneBytesRead"] + 20 * st["visible"]
+        st["algorithmic_gbs"] = alg / st["elapsedSeconds"] / 1e9 if st["elapsedSeconds"] > 0 else 0.0
+        st["hbm_frac"] = st["algorithmic_gbs"] / HBM_PEAK_GBS
         runs.append(st)
         log(f"served128 @ {rate:.0f} req/s: {st['requestsPerSecond']:.0f} req/s, "
             f"{st['drawablesPerSecond'] / 1e9:.1f} G tests/s, p50 {st['p50Ms']:.3f} ms, p99 {st['p99Ms']:.3f} ms, "
-            f"p999 {st['p999Ms']:.3f} ms, {st['meanViewsPerLaunch']:.1f} views/launch")
+            f"p999 {st['p999Ms']:.3f} ms, {st['meanViewsPerLaunch']:.1f} views/launch, "
+            f"{st['algorithmic_gbs']:.0f} GB/s algorithmic ({100 * st['hbm_frac']:.0f}% of HBM)")
     # saturation: the highest offered rate whose p99 stays under 2 ms
     under = [r["offered_rate"] for r in runs if r["p99Ms"] < 2.0 and r["refused"] == 0]
     srv.close()

@@ -124,7 +124,8 @@ class LoadStats(ctypes.Structure):
                 ("requestsPerSecond", ctypes.c_double), ("drawablesPerSecond", ctypes.c_double),
                 ("p50Ms", ctypes.c_double), ("p99Ms", ctypes.c_double), ("p999Ms", ctypes.c_double),
                 ("maxMs", ctypes.c_double), ("meanMs", ctypes.c_double), ("visible", c_u64), ("launches", c_u64),
-                ("meanViewsPerLaunch", ctypes.c_double), ("sampled", c_u64), ("failed", c_u64)]
+                ("meanViewsPerLaunch", ctypes.c_double), ("sampled", c_u64), ("failed", c_u64),
+                ("sceneBytesRead", c_u64)]
 
 
 class CadrError(RuntimeError):

@@ -174,6 +174,7 @@ struct cadrServer {
 	uint64_t wake = 0;          // bumped by submissions and releases
 	uint64_t wakeSeen = 0;      // value of wake at the last launch attempt
 	std::atomic<uint64_t> launches{0}, launchedViews{0};
+	std::atomic<uint64_t> sceneBytesLaunched{0};  // scene arrays each launch reads once per job
 	std::string workerMessage;  // first device failure of a launch (guarded by m)
 
 	~cadrServer();
@@ -369,6 +370,11 @@ bool cadrServer::launch(Slot& s)
 	okLaunch = okLaunch && hipEventRecord(s.done, s.stream) == hipSuccess;
 	launches++;
 	launchedViews += nv;
+	{
+		uint64_t sceneBytes = 0;  // spheres (16 B) and mesh indices (4 B) of every job's drawables
+		for (uint32_t j = 0; j < numJobs; j++) sceneBytes += uint64_t(s.hJobs[j].numDrawables) * 20u;
+		sceneBytesLaunched += sceneBytes;
+	}
 	if (!okLaunch) {
 		// nothing of this launch may be trusted: its requests complete now, failed
 		recordFailure(std::string("serve: a launch could not be issued: ") + cadrGetLastErrorMessage());
@@ -617,7 +623,7 @@ CADR_API int cadrServerRunLoad(cadrServer* s, const cadrLoadSpec* spec, cadrLoad
 	const uint32_t S = uint32_t(s->scenes.size());
 	const uint32_t hot = std::min(spec->numHotScenes, S);
 	*stats = cadrLoadStats{};
-	const uint64_t launches0 = s->launches, views0 = s->launchedViews;
+	const uint64_t launches0 = s->launches, views0 = s->launchedViews, sceneBytes0 = s->sceneBytesLaunched;
 
 	// arrival schedule (Poisson process) and request contents
 	uint64_t rng = spec->seed;
@@ -710,6 +716,7 @@ CADR_API int cadrServerRunLoad(cadrServer* s, const cadrLoadSpec* spec, cadrLoad
 	stats->completed = latMs.size();
 	stats->visible = visible;
 	stats->launches = s->launches - launches0;
+	stats->sceneBytesRead = s->sceneBytesLaunched - sceneBytes0;
 	stats->meanViewsPerLaunch = stats->launches ? double(s->launchedViews - views0) / double(stats->launches) : 0.0;
 	if (!latMs.empty()) {
 		std::sort(latMs.begin(), latMs.end());

@@ -685,6 +685,9 @@ typedef struct cadrLoadStats {
 	double meanViewsPerLaunch;
 	uint64_t sampled;               /* entries written to spec->samples          */
 	uint64_t failed;                /* completed with a non-zero status          */
+	uint64_t sceneBytesRead;        /* scene arrays the run's launches read: 20 B
+	                                 * per drawable per job (a scene's drawables
+	                                 * are read once for all its views in a launch) */
 } cadrLoadStats;
 
 int cadrServerRunLoad(cadrServer* server, const cadrLoadSpec* spec, cadrLoadStats* stats);
