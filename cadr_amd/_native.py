@@ -166,6 +166,12 @@ SIGNATURES = {
     "cadrPropagateTransforms": (c_int, [c_void_p, c_void_p]),
     "cadrPropagateTransformsBatch": (c_int, [c_u64, c_void_p, c_u32, c_void_p]),
     "cadrTransformPlanCreate": (c_int, [c_void_p, c_u32, c_void_p, ctypes.POINTER(c_void_p)]),
+    "cadrSharedBufferCreate": (c_int, [c_u64, ctypes.POINTER(c_void_p)]),
+    "cadrSharedBufferImport": (c_int, [c_int, c_u64, ctypes.POINTER(c_void_p)]),
+    "cadrSharedBufferAddress": (c_u64, [c_void_p]),
+    "cadrSharedBufferSize": (c_u64, [c_void_p]),
+    "cadrSharedBufferExportFd": (c_int, [c_void_p, ctypes.POINTER(c_int)]),
+    "cadrSharedBufferDestroy": (None, [c_void_p]),
     "cadrPropagateTransformsPlanned": (c_int, [c_void_p, c_void_p]),
     "cadrTransformPlanDestroy": (None, [c_void_p]),
     "cadrComputeWorldSpheres": (c_int, [c_u64, c_u64, c_u64, c_u32, c_u64, c_u64, c_void_p]),

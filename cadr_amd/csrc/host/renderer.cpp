@@ -399,8 +399,77 @@ void Renderer::growCull(uint64_t& buffer, size_t& capacity, size_t bytes, const 
 	capacity = cap;
 }
 
+// The same growth rule for a buffer that a Vulkan renderer imports: a
+// shared buffer (cadrSharedBufferCreate), replaced once the device is idle,
+// the generation telling the renderer to import it again.
+void Renderer::growCullShared(uint64_t& buffer, size_t& capacity, cadrSharedBuffer*& shared, size_t bytes,
+                              const char* what)
+{
+	if (bytes <= capacity && buffer) return;
+	if (shared) {
+		hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+		cadrSharedBufferDestroy(shared);
+	} else if (buffer) {
+		(void)hipFree(reinterpret_cast<void*>(buffer));
+	}
+	shared = nullptr;
+	buffer = 0;
+	capacity = 0;
+	const size_t cap = std::max<size_t>(256, bytes + bytes / 5);
+	throwOnError(cadrSharedBufferCreate(cap, &shared), what);
+	buffer = cadrSharedBufferAddress(shared);
+	capacity = size_t(cadrSharedBufferSize(shared));
+	_cullGeneration++;
+}
+
+void Renderer::setCullBuffersShared(bool on)
+{
+	if (on == _cullShared) return;
+	if (_device < 0 && on)
+		throw LogicError("CadR::Renderer::setCullBuffersShared(): a device-less renderer shares nothing");
+	// the next cull frame allocates the output buffers of the new kind
+	if (_device >= 0) hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+	for (auto bc : {std::make_pair(&_cull.commands, &_cull.sharedCommands),
+	                std::make_pair(&_cull.counts, &_cull.sharedCounts),
+	                std::make_pair(&_cull.pointers, &_cull.sharedPointers)}) {
+		if (*bc.second) cadrSharedBufferDestroy(*bc.second);
+		else if (*bc.first) (void)hipFree(reinterpret_cast<void*>(*bc.first));
+		*bc.first = 0;
+		*bc.second = nullptr;
+	}
+	_cull.commandsCap = _cull.countsCap = _cull.pointersCap = 0;
+	_cullShared = on;
+	_cullGeneration++;
+}
+
+int Renderer::exportCullFd(const cadrSharedBuffer* shared, const char* what) const
+{
+	if (!_cullShared || !shared)
+		throw LogicError(std::string("CadR::Renderer::") + what +
+		                 "(): no shared buffer (setCullBuffersShared, then a cull frame)");
+	int fd = -1;
+	throwOnError(cadrSharedBufferExportFd(shared, &fd), what);
+	return fd;
+}
+
+int Renderer::exportCullCommandBufferFd() const { return exportCullFd(_cull.sharedCommands, "exportCullCommandBufferFd"); }
+int Renderer::exportCullCountBufferFd() const { return exportCullFd(_cull.sharedCounts, "exportCullCountBufferFd"); }
+int Renderer::exportCullPointersBufferFd() const
+{
+	return exportCullFd(_cullPointersOn ? _cull.sharedPointers : nullptr, "exportCullPointersBufferFd");
+}
+
 void Renderer::releaseCull() noexcept
 {
+	// shared output buffers are released as such, the rest with hipFree
+	for (auto bc : {std::make_pair(&_cull.commands, &_cull.sharedCommands),
+	                std::make_pair(&_cull.counts, &_cull.sharedCounts),
+	                std::make_pair(&_cull.pointers, &_cull.sharedPointers)}) {
+		if (!*bc.second) continue;
+		cadrSharedBufferDestroy(*bc.second);
+		*bc.second = nullptr;
+		*bc.first = 0;
+	}
 	for (uint64_t* b : {&_cull.spheres, &_cull.meshIndex, &_cull.itemBase, &_cull.itemSource, &_cull.rangeBase,
 	                    &_cull.tiles, &_cull.views, &_cull.job, &_cull.commands, &_cull.counts, &_cull.pointers,
 	                    &_cull.sources, &_cull.meshes})
@@ -567,10 +636,20 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 	}
 
 	// outputs, views, tiles and the job
-	growCull(_cull.commands, _cull.commandsCap, std::max<size_t>(V * items * 20, 4), "hipMalloc(cull commands)");
-	growCull(_cull.counts, _cull.countsCap, V * K * 4, "hipMalloc(cull counts)");
-	if (_cullPointersOn)
-		growCull(_cull.pointers, _cull.pointersCap, std::max<size_t>(V * items * 32, 16), "hipMalloc(cull pointers)");
+	if (_cullShared) {
+		growCullShared(_cull.commands, _cull.commandsCap, _cull.sharedCommands, std::max<size_t>(V * items * 20, 4),
+		               "cadrSharedBufferCreate(cull commands)");
+		growCullShared(_cull.counts, _cull.countsCap, _cull.sharedCounts, std::max<size_t>(V * K * 4, 4),
+		               "cadrSharedBufferCreate(cull counts)");
+		if (_cullPointersOn)
+			growCullShared(_cull.pointers, _cull.pointersCap, _cull.sharedPointers,
+			               std::max<size_t>(V * items * 32, 16), "cadrSharedBufferCreate(cull pointers)");
+	} else {
+		growCull(_cull.commands, _cull.commandsCap, std::max<size_t>(V * items * 20, 4), "hipMalloc(cull commands)");
+		growCull(_cull.counts, _cull.countsCap, V * K * 4, "hipMalloc(cull counts)");
+		if (_cullPointersOn)
+			growCull(_cull.pointers, _cull.pointersCap, std::max<size_t>(V * items * 32, 16), "hipMalloc(cull pointers)");
+	}
 	growCull(_cull.views, _cull.viewsCap, V * sizeof(cadrView), "hipMalloc(cull views)");
 	std::memcpy(viewsHost, views, V * sizeof(cadrView));
 	hipCheck(hipMemcpyAsync(reinterpret_cast<void*>(_cull.views), viewsHost, V * sizeof(cadrView),

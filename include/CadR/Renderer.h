@@ -141,6 +141,20 @@ public:
 	void recordDrawableCulling(void* stream, const cadrView* views, size_t numViews);
 	void setCullConfig(const cadrCullConfig& singleView, const cadrCullConfig& batched);
 	void setCullDrawablePointers(bool on) { _cullPointersOn = on; }
+	// the command, count and DrawablePointers buffers of the cull frame as
+	// shared buffers a Vulkan renderer imports (cadrSharedBuffer*): each
+	// exportCull*Fd returns a new dma-buf descriptor the caller owns;
+	// cullBufferGeneration() changes whenever they are reallocated (growth),
+	// and the renderer imports them again
+	void setCullBuffersShared(bool on);
+	bool cullBuffersShared() const { return _cullShared; }
+	int exportCullCommandBufferFd() const;
+	int exportCullCountBufferFd() const;
+	int exportCullPointersBufferFd() const;
+	uint64_t cullBufferGeneration() const { return _cullGeneration; }
+	size_t cullCommandBufferSize() const { return _cull.commandsCap; }
+	size_t cullCountBufferSize() const { return _cull.countsCap; }
+	size_t cullPointersBufferSize() const { return _cullPointersOn ? _cull.pointersCap : 0; }
 	uint64_t cullCommandBufferAddress() const { return _cull.commands; }
 	uint64_t cullCountBufferAddress() const { return _cull.counts; }
 	uint64_t cullPointersBufferAddress() const { return _cullPointersOn ? _cull.pointers : 0; }
@@ -176,6 +190,9 @@ private:
 	void reserveDelta(size_t count);
 	void updateCullLayout(void* stream);
 	void growCull(uint64_t& buffer, size_t& capacity, size_t bytes, const char* what);
+	void growCullShared(uint64_t& buffer, size_t& capacity, cadrSharedBuffer*& shared, size_t bytes,
+	                    const char* what);
+	int exportCullFd(const cadrSharedBuffer* shared, const char* what) const;
 	void releaseCull() noexcept;
 
 	static Renderer* _defaultRenderer;
@@ -232,7 +249,10 @@ private:
 		uint64_t views = 0, job = 0, commands = 0, counts = 0, pointers = 0, sources = 0, meshes = 0;
 		size_t spheresCap = 0, meshIndexCap = 0, itemBaseCap = 0, itemSourceCap = 0, rangeBaseCap = 0, tilesCap = 0;
 		size_t viewsCap = 0, jobCap = 0, commandsCap = 0, countsCap = 0, pointersCap = 0, sourcesCap = 0, meshesCap = 0;
+		cadrSharedBuffer *sharedCommands = nullptr, *sharedCounts = nullptr, *sharedPointers = nullptr;
 	} _cull;
+	bool _cullShared = false;
+	uint64_t _cullGeneration = 0;
 	cadrCullWorkspace* _cullWs = nullptr;
 	uint64_t _cullWsWords = 0;
 	std::map<std::pair<const Geometry*, uint32_t>, uint32_t> _meshIndex;

@@ -188,6 +188,28 @@ int cadrProcessDrawables(const cadrProcessDrawablesPushConstants* pushConstants,
  * Renderer.cpp:635-644 staging -> device copy) by the records that changed;
  * the facade's Renderer uses it when the drawable order is unchanged.  The
  * order of writes to one slot listed twice is unspecified. */
+/* Device buffers shared with a Vulkan renderer through dma-buf file
+ * descriptors (VK_KHR_external_memory_fd): the cull's command, count and
+ * DrawablePointers outputs in memory the renderer's
+ * vkCmdDrawIndexedIndirectCount and shaders read directly.  Replaces the
+ * renderer-owned buffers of Renderer.cpp:446-595 read by gl_DrawID
+ * (RenderingPerformance/shader.vert:99-123).
+ *   Create: device memory allocated here (size rounded up to the allocation
+ *     granularity, cadrSharedBufferSize gives it), mapped at
+ *     cadrSharedBufferAddress; ExportFd returns a new descriptor for the
+ *     renderer's vkAllocateMemory + VkImportMemoryFdInfoKHR (the caller owns
+ *     it; a successful Vulkan import takes it over).
+ *   Import: memory the renderer allocated and exported (vkGetMemoryFdKHR),
+ *     mapped here; a successful import takes over fd.
+ * Destroy unmaps and releases this side's reference. */
+typedef struct cadrSharedBuffer cadrSharedBuffer;
+int cadrSharedBufferCreate(uint64_t size, cadrSharedBuffer** buffer);
+int cadrSharedBufferImport(int fd, uint64_t size, cadrSharedBuffer** buffer);
+uint64_t cadrSharedBufferAddress(const cadrSharedBuffer* buffer);
+uint64_t cadrSharedBufferSize(const cadrSharedBuffer* buffer);
+int cadrSharedBufferExportFd(const cadrSharedBuffer* buffer, int* fd);
+void cadrSharedBufferDestroy(cadrSharedBuffer* buffer);
+
 /* DrawablePointers for the cull's compacted draws, the reference-layout
  * companion of its commands.  The reference's shaders read
  * DrawablePointersRef[gl_DrawID] (CadPL/UberShaderPoints.vert:38) from a

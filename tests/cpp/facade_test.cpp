@@ -761,11 +761,14 @@ static void randomMatrix(std::mt19937& rng, float range, float* M)
 // matrices (one item per matrix, some lists written on the GPU by a
 // transform level); StateSets nested and one reached through two parents;
 // a Geometry destroyed while its drawables remain; views from cadrMakeView.
-static void gpuCullFrames(uint32_t seed, bool instancing, size_t numDrawables)
+static void gpuCullFrames(uint32_t seed, bool instancing, size_t numDrawables, bool shared = false)
 {
-	std::printf("gpu cull frames: seed %u, instancing %d, %zu drawables\n", seed, int(instancing), numDrawables);
+	std::printf("gpu cull frames: seed %u, instancing %d, %zu drawables, shared buffers %d\n", seed, int(instancing),
+	            numDrawables, int(shared));
 	std::fflush(stdout);
 	Renderer r(0, false);
+	r.setCullBuffersShared(shared);
+	uint64_t generation = r.cullBufferGeneration();
 	DataStorage& ds = r.dataStorage();
 	std::mt19937 rng(seed);
 	if (seed % 2) for (int i = 0; i < 3000; i++) ds.createHandle();  // handle level 2
@@ -992,6 +995,30 @@ static void gpuCullFrames(uint32_t seed, bool instancing, size_t numDrawables)
 			}
 		}
 		CHECK(visible > 0 && visible < V * items, "some items culled, some visible");
+		if (shared) {
+			// what a Vulkan renderer imports: the same bytes through the exported descriptors
+			CHECK(frame == 2 || r.cullBufferGeneration() != generation, "grown buffers are a new generation");
+			generation = r.cullBufferGeneration();
+			auto viaImport = [&](int fd, size_t size, void* dst, size_t bytes) {
+				cadrSharedBuffer* imp = nullptr;
+				CHECK(fd >= 0 && cadrSharedBufferImport(fd, size, &imp) == 0, "import of an exported cull buffer");
+				CHECK(hipMemcpy(dst, reinterpret_cast<void*>(cadrSharedBufferAddress(imp)), bytes,
+				                hipMemcpyDeviceToHost) == hipSuccess,
+				      "read through the import");
+				cadrSharedBufferDestroy(imp);
+			};
+			std::vector<cadrDrawIndexedIndirectCommand> impCmds(V * items);
+			viaImport(r.exportCullCommandBufferFd(), r.cullCommandBufferSize(), impCmds.data(), impCmds.size() * 20);
+			CHECK(std::memcmp(impCmds.data(), gotCmds.data(), gotCmds.size() * 20) == 0, "imported commands");
+			std::vector<uint32_t> impCounts(gotCounts.size());
+			viaImport(r.exportCullCountBufferFd(), r.cullCountBufferSize(), impCounts.data(), impCounts.size() * 4);
+			CHECK(impCounts == gotCounts, "imported counts");
+			if (frame > 0) {
+				std::vector<cadrDrawablePointers> impPtr(V * items);
+				viaImport(r.exportCullPointersBufferFd(), r.cullPointersBufferSize(), impPtr.data(), impPtr.size() * 32);
+				CHECK(std::memcmp(impPtr.data(), gotPtr.data(), gotPtr.size() * 32) == 0, "imported DrawablePointers");
+			}
+		}
 	}
 	(void)hipStreamDestroy(stream);
 	for (void* p : deviceArrays) (void)hipFree(p);
@@ -1030,6 +1057,7 @@ int main(int argc, char** argv)
 			gpuResidentFrames(5);
 			gpuCullFrames(6, false, 20000);
 			gpuCullFrames(7, true, 8000);
+			gpuCullFrames(8, true, 3000, true);
 		} else {
 			std::fprintf(stderr, "usage: facade_test cpu|gpu\n");
 			return 2;
