@@ -116,24 +116,23 @@ def parse():
     return p.parse_args()
 
 
-class Timer:
-    """HIP events around one kernel launch on the stream, summed over steps."""
+class KernelEnds:
+    """One HIP event per timed step that a kernel launch records at the end of
+    its own dispatch (cadrTimeNextLaunch with a stop event only).  A kernel's
+    time in a chain of launches on one stream is its end minus the end of the
+    launch before it: its execution plus the dispatch gap in front of it
+    (under a microsecond), an upper bound on its duration.  Stop events bound
+    to a dispatch add nothing to the chain; an event recorded as a command of
+    its own (or a start event) adds about 4 us per kernel."""
 
     def __init__(self, n):
         from cadr_amd.device import Event
-        self.starts = [Event() for _ in range(n)]
         self.ends = [Event() for _ in range(n)]
-        self.i = 0
 
-    def begin(self, stream):
-        self.starts[self.i].record(stream)
-
-    def end(self, stream):
-        self.ends[self.i].record(stream)
-        self.i += 1
-
-    def mean_ms(self):
-        return sum(s.elapsed_ms(e) for s, e in zip(self.starts[: self.i], self.ends[: self.i])) / max(self.i, 1)
+    @staticmethod
+    def mean_ms(before: "KernelEnds", after: "KernelEnds") -> float:
+        n = len(after.ends)
+        return sum(b.elapsed_ms(a) for b, a in zip(before.ends, after.ends)) / max(n, 1)
 
 
 def main():
@@ -178,7 +177,7 @@ def main():
     level_descs = ds.level_descs(spheres=True)
     leaf_fused = level_descs[-1].copy()
     leaf_fused["spheres"] = 0  # the one-view frame needs no world spheres in HBM
-    from cadr_amd.device import Event
+    from cadr_amd.device import Event, time_next_launch
 
     # the levels above the leaf level; a root level's world matrices are its
     # local ones, so the level below it reads the root's local matrices as its
@@ -191,10 +190,17 @@ def main():
         first_pair_dev = device.DeviceBuffer.from_array(first_pair)
         upper = upper[2:]
 
-    def propagate_inner():
+    def propagate_inner(end=None):
+        """end: event that the last of these launches records at its end"""
+        if end is not None and first_pair is None and not upper:
+            end.record(s)
         if first_pair is not None:
+            if end is not None and not upper:
+                time_next_launch(None, end)
             N.check(lib.cadrPropagateTransformsBatch(first_pair_dev.address, N.ptr(first_pair), 2, s))
-        for lv in upper:
+        for i, lv in enumerate(upper):
+            if end is not None and i == len(upper) - 1:
+                time_next_launch(None, end)
             N.check(lib.cadrPropagateTransforms(N.ptr(lv), s))
 
     # kernel shape of the fused leaf + cull launch: the fastest of the sweep
@@ -219,40 +225,37 @@ def main():
         log(f"fused autotune: {best}  {fused_table}")
     batch_fused = cull.CullBatch(fused_config, [(ds, views1, out1)])
 
-    total = args.warmup + args.steps
-
     def run_frames(fused):
         """warmup + steps frames; returns (ms per frame, mean ms of the leaf
         launch, mean ms of the separate cull launch or None, counts)"""
-        t_leaf, t_cull = Timer(total), Timer(total)
+        e_inner, e_leaf, e_cull = KernelEnds(args.steps), KernelEnds(args.steps), KernelEnds(args.steps)
 
-        def frame():
-            propagate_inner()
-            t_leaf.begin(s)
+        def frame(k=None):
+            """k: index of a timed frame, whose launches record their ends"""
+            propagate_inner(None if k is None else e_inner.ends[k])
+            if k is not None:
+                time_next_launch(None, e_leaf.ends[k])
             if fused:
                 batch_fused.run_fused(leaf_fused, s)
             else:
                 N.check(lib.cadrPropagateTransforms(N.ptr(level_descs[-1]), s))
-            t_leaf.end(s)
-            if not fused:
-                t_cull.begin(s)
+                if k is not None:
+                    time_next_launch(None, e_cull.ends[k])
                 batch1.run_view(s)
-                t_cull.end(s)
 
         for _ in range(args.warmup):
             frame()
         stream.synchronize()
         # the timed region: exactly `steps` frames, synchronised on both sides
-        for t in (t_leaf, t_cull):
-            t.starts, t.ends, t.i = t.starts[args.warmup:], t.ends[args.warmup:], 0
         device.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            frame()
+        for k in range(args.steps):
+            frame(k)
         stream.synchronize()
         device.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / args.steps
-        return ms, t_leaf.mean_ms(), (None if fused else t_cull.mean_ms()), out1.download_counts()
+        return (ms, KernelEnds.mean_ms(e_inner, e_leaf), (None if fused else KernelEnds.mean_ms(e_leaf, e_cull)),
+                out1.download_counts())
 
     un_ms, leaf_ms, cull_ms, counts_un = run_frames(False)
     ms_per_step, fused_ms, _, counts1 = run_frames(True)
@@ -297,6 +300,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": f"cullFusedKernel<{fused_config.blockSize},{fused_config.itemsPerThread}> "
                                                "(leaf propagation + cull + LOD + compaction)",
                      "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fused_gbs / HBM_PEAK_GBS,
+                     "duration": "mean over the timed frames of the kernel's end minus the end of the launch "
+                                 "before it, both from events its dispatches record (cadrTimeNextLaunch)",
                      "traffic": measured_traffic(f"cullFusedKernel<{fused_config.blockSize},{fused_config.itemsPerThread}>"),
                      "algorithmic_bytes": fused_bytes, "avg_ms": fused_ms},
         "frame": {"algorithmic_bytes": frame_bytes, "hbm_frac": frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -161,6 +161,7 @@ SIGNATURES = {
     "cadrEventSynchronize": (c_int, [c_void_p]),
     "cadrEventQuery": (c_int, [c_void_p]),
     "cadrEventElapsedTime": (c_int, [ctypes.POINTER(c_float), c_void_p, c_void_p]),
+    "cadrTimeNextLaunch": (c_int, [c_void_p, c_void_p]),
     "cadrProcessDrawables": (c_int, [ctypes.POINTER(PushConstants), c_u32, c_u64, c_void_p]),
     "cadrScatterDrawables": (c_int, [c_u64, c_u64, c_u64, c_u64, c_u32, c_void_p]),
     "cadrPropagateTransforms": (c_int, [c_void_p, c_void_p]),

@@ -35,6 +35,7 @@
 //     per-drawable test when the tile is wholly outside or inside the
 //     frustum.
 #include "internal.h"
+#include "launch.h"
 #include "fp32.h"
 #include "matrix_io.h"
 
@@ -1443,7 +1444,7 @@ int launchBatch(KernelFn fn, const KernelEntry& ke, const cadrCullConfig* config
                 const cadrTransformLevel* leaves, const std::string& name)
 {
 	CADR_REQUIRE(ws, name + ": workspace is NULL");
-	if (numJobs == 0) return ok();
+	if (numJobs == 0) CADR_RETURN_NO_LAUNCH(stream);
 	CADR_REQUIRE(deviceJobs && hostJobs, name + ": null job array");
 	CADR_REQUIRE(deviceJobs % 16 == 0, name + ": deviceJobs must be 16-byte aligned");
 	CADR_REQUIRE(numJobs <= ws->maxJobs, name + ": more jobs than the workspace was created for");
@@ -1472,7 +1473,7 @@ int launchBatch(KernelFn fn, const KernelEntry& ke, const cadrCullConfig* config
 	hipLaunchKernelGGL(cullSetupKernel, dim3(1), dim3(256), 0, s, jobs, numJobs, splitTiles, ws->firstWork,
 	                   ws->statusBase, ws->ctl);
 	CADR_CHECK_HIP(hipGetLastError(), "cullSetupKernel launch");
-	if (totalWork == 0) return ok();
+	if (totalWork == 0) CADR_RETURN_NO_LAUNCH(s);
 	if (grid > totalWork) grid = totalWork;
 	CullParams p{};
 	p.jobs = jobs;
@@ -1486,7 +1487,7 @@ int launchBatch(KernelFn fn, const KernelEntry& ke, const cadrCullConfig* config
 	p.flags = config->flags;
 	p.waitLimit = ws->waitLimit;
 	p.leaves = leaves;
-	hipLaunchKernelGGL(fn, dim3(uint32_t(grid)), dim3(ke.block), 0, s, p);
+	launchKernel(fn, dim3(uint32_t(grid)), dim3(ke.block), s, p);
 	CADR_CHECK_HIP(hipGetLastError(), "cullKernel launch");
 	return ok();
 }
@@ -1508,7 +1509,7 @@ CADR_API int cadrPropagateCullBatch(uint64_t deviceLeaves, const cadrTransformLe
 	const KernelEntry* ke = findKernel(config);
 	CADR_REQUIRE(ke && ke->fusedBatch,
 	             "cadrPropagateCullBatch: unsupported blockSize/itemsPerThread (256 x 4/8, 512 x 4)");
-	if (numJobs == 0) return ok();
+	if (numJobs == 0) CADR_RETURN_NO_LAUNCH(stream);
 	CADR_REQUIRE(deviceLeaves && hostLeaves && hostJobs, "cadrPropagateCullBatch: null leaf or job array");
 	CADR_REQUIRE(deviceLeaves % 8 == 0, "cadrPropagateCullBatch: deviceLeaves must be 8-byte aligned");
 	for (uint32_t j = 0; j < numJobs; j++) {
@@ -1534,7 +1535,7 @@ CADR_API int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, cons
 	int dev = 0;
 	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
 	CADR_REQUIRE(dev == ws->device, "cadrCullView: workspace belongs to another device");
-	if (hostJob->numTiles == 0) return ok();
+	if (hostJob->numTiles == 0) CADR_RETURN_NO_LAUNCH(stream);
 	CullParams p{};
 	p.jobs = reinterpret_cast<const cadrCullJob*>(deviceJob);
 	p.status = ws->status;
@@ -1543,7 +1544,7 @@ CADR_API int cadrCullView(const cadrCullConfig* config, uint64_t deviceJob, cons
 	p.totalWork = hostJob->numTiles;
 	p.flags = config->flags;
 	p.waitLimit = ws->waitLimit;
-	hipLaunchKernelGGL(ke->view, dim3(hostJob->numTiles), dim3(ke->block), 0, hipStream_t(stream), p);
+	launchKernel(ke->view, dim3(hostJob->numTiles), dim3(ke->block), hipStream_t(stream), p);
 	CADR_CHECK_HIP(hipGetLastError(), "cullViewKernel launch");
 	return ok();
 }
@@ -1566,7 +1567,7 @@ CADR_API int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCul
 	int dev = 0;
 	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
 	CADR_REQUIRE(dev == ws->device, "cadrPropagateCullView: workspace belongs to another device");
-	if (hostJob->numTiles == 0) return ok();
+	if (hostJob->numTiles == 0) CADR_RETURN_NO_LAUNCH(stream);
 	CullParams p{};
 	p.jobs = reinterpret_cast<const cadrCullJob*>(deviceJob);
 	p.status = ws->status;
@@ -1577,7 +1578,7 @@ CADR_API int cadrPropagateCullView(const cadrTransformLevel* leaf, const cadrCul
 	p.waitLimit = ws->waitLimit;
 	p.leaf = LeafParams{leaf->localMatrices, leaf->parentIndex, leaf->parentWorld, leaf->worldMatrices, leaf->spheres,
 	                    leaf->parentWorld ? leaf->parentCount : 0u};
-	hipLaunchKernelGGL(ke->fused, dim3(hostJob->numTiles), dim3(ke->block), 0, hipStream_t(stream), p);
+	launchKernel(ke->fused, dim3(hostJob->numTiles), dim3(ke->block), hipStream_t(stream), p);
 	CADR_CHECK_HIP(hipGetLastError(), "fused cullViewKernel launch");
 	return ok();
 }
@@ -1649,7 +1650,7 @@ CADR_API int cadrCullBatchNoAlloc(const cadrCullConfig* config, uint64_t deviceJ
 {
 	const KernelEntry* ke = findKernel(config);
 	CADR_REQUIRE(ke && ke->batch, "cadrCullBatchNoAlloc: unsupported blockSize/itemsPerThread");
-	if (numJobs == 0) return ok();
+	if (numJobs == 0) CADR_RETURN_NO_LAUNCH(stream);
 	CADR_REQUIRE(hostJobs, "cadrCullBatchNoAlloc: hostJobs is NULL");
 	if (numJobs > kScratchMaxJobs)
 		return setError(CADR_ERROR_INVALID_VALUE, "cadrCullBatchNoAlloc: " + std::to_string(numJobs) +

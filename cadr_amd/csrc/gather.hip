@@ -14,6 +14,7 @@
 // with 16-byte stores; consecutive lanes touch consecutive records, so every
 // wave reads and writes whole contiguous spans.
 #include "internal.h"
+#include "launch.h"
 #include "handles.h"
 
 namespace cadr {
@@ -120,7 +121,7 @@ CADR_API int cadrCompactDrawablePointers(uint64_t commands, uint64_t counts, uin
                                          uint32_t numStateSets, uint32_t commandStride, uint64_t pointersIn,
                                          uint64_t numDrawables, uint64_t pointersOut, cadrStream stream)
 {
-	if (numViews == 0 || numStateSets == 0) return ok();
+	if (numViews == 0 || numStateSets == 0) CADR_RETURN_NO_LAUNCH(stream);
 	CADR_REQUIRE(commands && counts && stateSetBase && pointersIn && pointersOut,
 	             "cadrCompactDrawablePointers: null device address");
 	CADR_REQUIRE((commands % 4) == 0 && (counts % 4) == 0 && (stateSetBase % 4) == 0 && (pointersIn % 16) == 0 &&
@@ -131,11 +132,10 @@ CADR_API int cadrCompactDrawablePointers(uint64_t commands, uint64_t counts, uin
 	const uint32_t split = uint32_t(std::max<uint64_t>(1, (avg + 2047) / 2048));
 	CADR_REQUIRE(uint64_t(numViews) * numStateSets * split < (1ull << 31),
 	             "cadrCompactDrawablePointers: too many (view, StateSet) segments");
-	hipLaunchKernelGGL(compactPointersKernel, dim3(numViews * numStateSets * split), dim3(256), 0,
-	                   hipStream_t(stream), reinterpret_cast<const uint32_t*>(commands),
-	                   reinterpret_cast<const uint32_t*>(counts), reinterpret_cast<const uint32_t*>(stateSetBase),
-	                   numStateSets, commandStride, split,
-	                   reinterpret_cast<const uint4*>(pointersIn), numDrawables, reinterpret_cast<uint4*>(pointersOut));
+	launchKernel(compactPointersKernel, dim3(numViews * numStateSets * split), dim3(256), hipStream_t(stream),
+	             reinterpret_cast<const uint32_t*>(commands), reinterpret_cast<const uint32_t*>(counts),
+	             reinterpret_cast<const uint32_t*>(stateSetBase), numStateSets, commandStride, split,
+	             reinterpret_cast<const uint4*>(pointersIn), numDrawables, reinterpret_cast<uint4*>(pointersOut));
 	CADR_CHECK_HIP(hipGetLastError(), "compactPointersKernel launch");
 	return ok();
 }
@@ -158,7 +158,7 @@ CADR_API int cadrProcessDrawables(const cadrProcessDrawablesPushConstants* pc, u
                                   uint64_t numDrawables, cadrStream stream)
 {
 	CADR_REQUIRE(pc, "cadrProcessDrawables: pushConstants is NULL");
-	if (numDrawables == 0) return ok();  // Renderer.cpp:600-620 returns early
+	if (numDrawables == 0) CADR_RETURN_NO_LAUNCH(stream);  // Renderer.cpp:600-620 returns early
 	CADR_REQUIRE(numDrawables <= CADR_MAX_DRAWABLES,
 	             "cadrProcessDrawables: numDrawables exceeds the 1Gi limit (Renderer.cpp:687)");
 	CADR_REQUIRE(handleLevel >= 1 && handleLevel <= 3, "cadrProcessDrawables: handleLevel must be 1, 2 or 3");
@@ -174,11 +174,9 @@ CADR_API int cadrProcessDrawables(const cadrProcessDrawablesPushConstants* pc, u
 	auto* indirect = reinterpret_cast<uint4*>(pc->indirectData);
 	auto* pointers = reinterpret_cast<uint4*>(pc->drawablePointers);
 	hipStream_t s = hipStream_t(stream);
-	switch (handleLevel) {
-	case 1: hipLaunchKernelGGL(processDrawablesKernel<1>, dim3(uint32_t(blocks)), dim3(256), 0, s, table, list, indirect, pointers, numDrawables); break;
-	case 2: hipLaunchKernelGGL(processDrawablesKernel<2>, dim3(uint32_t(blocks)), dim3(256), 0, s, table, list, indirect, pointers, numDrawables); break;
-	default: hipLaunchKernelGGL(processDrawablesKernel<3>, dim3(uint32_t(blocks)), dim3(256), 0, s, table, list, indirect, pointers, numDrawables); break;
-	}
+	auto kernel = handleLevel == 1 ? processDrawablesKernel<1>
+	              : handleLevel == 2 ? processDrawablesKernel<2> : processDrawablesKernel<3>;
+	launchKernel(kernel, dim3(uint32_t(blocks)), dim3(256), s, table, list, indirect, pointers, numDrawables);
 	CADR_CHECK_HIP(hipGetLastError(), "processDrawablesKernel launch");
 	return ok();
 }

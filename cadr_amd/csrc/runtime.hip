@@ -5,6 +5,7 @@
 // (src/CadR/Renderer.cpp:97-313 init, :946-999 executeCopyOperations,
 // src/CadR/DataMemory.cpp:255-397 block allocation).
 #include "internal.h"
+#include "launch.h"
 
 #include <mutex>
 #include <vector>
@@ -12,6 +13,23 @@
 namespace cadr {
 
 static thread_local std::string g_lastError;
+static thread_local LaunchEvents g_launchEvents;
+
+LaunchEvents takeLaunchEvents()
+{
+	const LaunchEvents e = g_launchEvents;
+	g_launchEvents = LaunchEvents{};
+	return e;
+}
+
+hipError_t recordSkippedLaunch(hipStream_t s)
+{
+	const LaunchEvents e = takeLaunchEvents();
+	hipError_t err = hipSuccess;
+	if (e.start) err = hipEventRecord(e.start, s);
+	if (err == hipSuccess && e.stop) err = hipEventRecord(e.stop, s);
+	return err;
+}
 
 int setError(int code, const std::string& message)
 {
@@ -244,6 +262,12 @@ CADR_API int cadrEventQuery(cadrEvent event)
 	hipError_t e = hipEventQuery(hipEvent_t(event));
 	if (e == hipErrorNotReady) return CADR_ERROR_NOT_READY;
 	CADR_CHECK_HIP(e, "hipEventQuery");
+	return ok();
+}
+
+CADR_API int cadrTimeNextLaunch(cadrEvent start, cadrEvent stop)
+{
+	g_launchEvents = LaunchEvents{hipEvent_t(start), hipEvent_t(stop)};
 	return ok();
 }
 

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "launch.h"
 #include "fp32.h"
 #include "matrix_io.h"
 
@@ -241,7 +242,7 @@ using namespace cadr;
 CADR_API int cadrPropagateTransforms(const cadrTransformLevel* lv, cadrStream stream)
 {
 	CADR_REQUIRE(lv, "cadrPropagateTransforms: level is NULL");
-	if (lv->count == 0) return ok();
+	if (lv->count == 0) CADR_RETURN_NO_LAUNCH(stream);
 	CADR_REQUIRE(lv->localMatrices && lv->worldMatrices, "cadrPropagateTransforms: null matrix array");
 	CADR_REQUIRE((lv->localMatrices % 16) == 0 && (lv->worldMatrices % 16) == 0 && (lv->parentWorld % 16) == 0,
 	             "cadrPropagateTransforms: matrix arrays must be 16-byte aligned");
@@ -261,10 +262,10 @@ CADR_API int cadrPropagateTransforms(const cadrTransformLevel* lv, cadrStream st
 	auto* sph = reinterpret_cast<float4*>(lv->spheres);
 	hipStream_t s = hipStream_t(stream);
 	const dim3 g{uint32_t(blocks)}, b{256};
-	if (root && spheres) hipLaunchKernelGGL((propagateKernel<true, true>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
-	else if (root) hipLaunchKernelGGL((propagateKernel<true, false>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
-	else if (spheres) hipLaunchKernelGGL((propagateKernel<false, true>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
-	else hipLaunchKernelGGL((propagateKernel<false, false>), g, b, 0, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes, sph);
+	auto kernel = root ? (spheres ? propagateKernel<true, true> : propagateKernel<true, false>)
+	                   : (spheres ? propagateKernel<false, true> : propagateKernel<false, false>);
+	launchKernel(kernel, g, b, s, local, parent, pworld, lv->parentCount, world, lv->count, mi, meshes, lv->numMeshes,
+	             sph);
 	CADR_CHECK_HIP(hipGetLastError(), "propagateKernel launch");
 	return ok();
 }
@@ -298,12 +299,11 @@ CADR_API int cadrPropagateTransformsBatch(uint64_t deviceLevels, const cadrTrans
 		units += (hostLevels[l].count + 63u) / 64u;
 		spheres = spheres || hostLevels[l].spheres != 0;
 	}
-	if (units == 0) return ok();
+	if (units == 0) CADR_RETURN_NO_LAUNCH(stream);
 	// without world spheres the kernel holds fewer registers: 8 waves per SIMD instead of 5
 	const uint64_t blocks = std::min<uint64_t>((units + 3) / 4, uint64_t(computeUnitCount()) * 16u);
-	hipLaunchKernelGGL(spheres ? propagateBatchKernel<true> : propagateBatchKernel<false>, dim3(uint32_t(blocks)),
-	                   dim3(256), 0, hipStream_t(stream), reinterpret_cast<const cadrTransformLevel*>(deviceLevels),
-	                   numLevels);
+	launchKernel(spheres ? propagateBatchKernel<true> : propagateBatchKernel<false>, dim3(uint32_t(blocks)), dim3(256),
+	             hipStream_t(stream), reinterpret_cast<const cadrTransformLevel*>(deviceLevels), numLevels);
 	CADR_CHECK_HIP(hipGetLastError(), "propagateBatchKernel launch");
 	return ok();
 }
@@ -390,14 +390,14 @@ CADR_API int cadrTransformPlanCreate(const cadrTransformLevel* hostLevels, uint3
 CADR_API int cadrPropagateTransformsPlanned(const cadrTransformPlan* plan, cadrStream stream)
 {
 	CADR_REQUIRE(plan, "cadrPropagateTransformsPlanned: plan is NULL");
-	if (plan->numUnits == 0) return ok();
+	if (plan->numUnits == 0) CADR_RETURN_NO_LAUNCH(stream);
 	int dev = 0;
 	CADR_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
 	CADR_REQUIRE(dev == plan->device, "cadrPropagateTransformsPlanned: the plan belongs to another device");
 	const uint64_t blocks = (plan->numUnits + 3) / 4;
 	CADR_REQUIRE(blocks < (1ull << 31), "cadrPropagateTransformsPlanned: plan too large");
-	hipLaunchKernelGGL(plan->spheres ? propagatePlanKernel<true> : propagatePlanKernel<false>, dim3(uint32_t(blocks)),
-	                   dim3(256), 0, hipStream_t(stream), plan->levels, plan->units, plan->numUnits);
+	launchKernel(plan->spheres ? propagatePlanKernel<true> : propagatePlanKernel<false>, dim3(uint32_t(blocks)),
+	             dim3(256), hipStream_t(stream), plan->levels, plan->units, plan->numUnits);
 	CADR_CHECK_HIP(hipGetLastError(), "propagatePlanKernel launch");
 	return ok();
 }

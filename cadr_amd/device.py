@@ -116,3 +116,10 @@ class Event:
         if self.handle:
             N.check(N.load().cadrEventDestroy(self.handle))
             self.handle = None
+
+
+def time_next_launch(start: "Event | None", stop: "Event | None") -> None:
+    """The calling thread's next pass launch records start / stop on its own
+    kernel dispatch (cadrTimeNextLaunch): the kernel's duration with no event
+    commands queued between kernels."""
+    N.check(N.load().cadrTimeNextLaunch(start.handle if start else None, stop.handle if stop else None))

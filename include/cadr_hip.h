@@ -86,6 +86,21 @@ int cadrEventSynchronize(cadrEvent event);
 /* CADR_SUCCESS when complete, CADR_ERROR_NOT_READY otherwise. */
 int cadrEventQuery(cadrEvent event);
 int cadrEventElapsedTime(float* milliseconds, cadrEvent start, cadrEvent stop);
+/* Kernel timestamps without marker commands: the next entry point this thread
+ * calls among cadrPropagateTransforms, cadrPropagateTransformsBatch,
+ * cadrPropagateTransformsPlanned, cadrProcessDrawables, cadrCullView,
+ * cadrPropagateCullView, cadrCullBatch, cadrPropagateCullBatch,
+ * cadrCullBatchNoAlloc and cadrCompactDrawablePointers records start and stop
+ * (either may be NULL) on the dispatch of its main kernel, so
+ * cadrEventElapsedTime(start, stop) is that kernel's own duration, and the
+ * stream carries no extra commands between kernels (an event recorded with
+ * cadrEventRecord is a command of its own and adds a few microseconds to a
+ * chain of kernels).  An entry point that launches nothing records both back
+ * to back; one that fails before launching leaves them pending.  The events
+ * stay pending until such a call; cadrTimeNextLaunch(NULL, NULL) clears them.
+ * Plays the part of the reference's vkCmdWriteTimestamp pair around its
+ * compute dispatch (src/CadR/Renderer.cpp:598-720). */
+int cadrTimeNextLaunch(cadrEvent start, cadrEvent stop);
 
 /* ------------------------------------------------------------------------ */
 /* Byte layouts shared with the reference (all little endian).              */
