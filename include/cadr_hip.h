@@ -99,7 +99,8 @@ int cadrEventElapsedTime(float* milliseconds, cadrEvent start, cadrEvent stop);
  * to back; one that fails before launching leaves them pending.  The events
  * stay pending until such a call; cadrTimeNextLaunch(NULL, NULL) clears them.
  * Plays the part of the reference's vkCmdWriteTimestamp pair around its
- * compute dispatch (src/CadR/Renderer.cpp:598-720). */
+ * compute dispatch (src/CadR/Renderer.cpp:660-700; both written at once
+ * when there is nothing to dispatch, :600-617). */
 int cadrTimeNextLaunch(cadrEvent start, cadrEvent stop);
 
 /* ------------------------------------------------------------------------ */
