@@ -220,12 +220,19 @@ void Renderer::recordDrawableProcessing(void* stream, size_t numDrawables)
 		uploadFullList(stream, numDrawables);
 	hipCheck(hipEventRecord(hipEvent_t(_uploadDone), s), "hipEventRecord");
 	_uploadPending = true;
-	recordEvent(1, stream);
+	// the timestamps around the dispatch (Renderer.cpp:660-700) are recorded
+	// by the dispatch itself: no marker commands between the upload and the
+	// gather
+	if (_collectFrameInfo)
+		throwOnError(cadrTimeNextLaunch(_events[1], _events[2]), "CadR::Renderer::recordDrawableProcessing");
 	const cadrProcessDrawablesPushConstants pc{_dataStorage->handleTableDeviceAddress(), _drawableBuffer,
 	                                           _indirectBuffer, _pointersBuffer};
-	throwOnError(cadrProcessDrawables(&pc, _dataStorage->handleLevel(), numDrawables, stream),
-	             "CadR::Renderer::recordDrawableProcessing");
-	recordEvent(2, stream);
+	const int rc = cadrProcessDrawables(&pc, _dataStorage->handleLevel(), numDrawables, stream);
+	if (rc != CADR_SUCCESS) {
+		if (_collectFrameInfo) (void)cadrTimeNextLaunch(nullptr, nullptr);
+		throwOnError(rc, "CadR::Renderer::recordDrawableProcessing");
+	}
+	if (_collectFrameInfo) _eventRecorded[1] = _eventRecorded[2] = true;
 	hipCheck(hipEventRecord(hipEvent_t(_listFree), s), "hipEventRecord");
 	_listFreePending = true;
 }
