@@ -76,17 +76,29 @@ __device__ __forceinline__ uint32_t stageSlot(uint32_t q)
 // registers through LDS: four 16-byte loads per lane at lane stride 16 B (each
 // instruction one contiguous KiB), then each lane reads its own 64 bytes from
 // LDS; stores go the same way in reverse, both as streaming accesses.  Matrix
-// `first` is lane 0's; lanes whose matrix index is at or past `count` neither
-// load nor store.
+// `first` is lane 0's; count >= 1.  The four loads are unconditional, so all
+// of them are in flight at once (a guarded load per instruction made the
+// compiler wait for each before issuing the next): lanes whose matrix index
+// is at or past `count` read matrix count - 1 again, and nothing of theirs is
+// stored.
 template <class Src>
-__device__ __forceinline__ void loadMatrices(Src src, uint64_t first, uint64_t count, float4* stage, uint32_t lane,
-                                             float4 M[4])
+__device__ __forceinline__ void issueMatrixLoads(Src src, uint64_t first, uint64_t count, uint32_t lane,
+                                                 float4 raw[4])
 {
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
 		const uint32_t q = uint32_t(k) * 64u + lane;      // float4 index inside the wave's 4 KiB
-		if (first + q / 4u < count) stage[stageSlot(q)] = loadF4Stream(src + (4 * first + q));
+		const uint64_t m = min(first + q / 4u, count - 1u);
+		raw[k] = loadF4Stream(src + (4 * m + (q & 3u)));
 	}
+}
+
+// The loaded block (issueMatrixLoads) redistributed through LDS: lane l gets
+// matrix first + l.
+__device__ __forceinline__ void stageMatrices(float4* stage, uint32_t lane, const float4 raw[4], float4 M[4])
+{
+#pragma unroll
+	for (int k = 0; k < 4; k++) stage[stageSlot(uint32_t(k) * 64u + lane)] = raw[k];
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 	__builtin_amdgcn_wave_barrier();
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -95,6 +107,15 @@ __device__ __forceinline__ void loadMatrices(Src src, uint64_t first, uint64_t c
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 	__builtin_amdgcn_wave_barrier();
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class Src>
+__device__ __forceinline__ void loadMatrices(Src src, uint64_t first, uint64_t count, float4* stage, uint32_t lane,
+                                             float4 M[4])
+{
+	float4 raw[4];
+	issueMatrixLoads(src, first, count, lane, raw);
+	stageMatrices(stage, lane, raw, M);
 }
 
 template <class Dst>
@@ -106,10 +127,13 @@ __device__ __forceinline__ void storeMatrices(Dst dst, uint64_t first, uint64_t 
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 	__builtin_amdgcn_wave_barrier();
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+	float4 out[4];  // all four LDS reads first, then the stores
+#pragma unroll
+	for (int k = 0; k < 4; k++) out[k] = stage[stageSlot(uint32_t(k) * 64u + lane)];
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
 		const uint32_t q = uint32_t(k) * 64u + lane;
-		if (first + q / 4u < count) storeF4Stream(dst + (4 * first + q), stage[stageSlot(q)]);
+		if (first + q / 4u < count) storeF4Stream(dst + (4 * first + q), out[k]);
 	}
 }
 

@@ -65,18 +65,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void p
 		return;
 	}
 	float4 L[4];
-	loadMatrices(local, first, count, stage[wave], lane, L);
 	if constexpr (ROOT) {
+		loadMatrices(local, first, count, stage[wave], lane, L);
 		storeMatrices(world, first, count, stage[wave], lane, L);
 		if (live) spheres[i] = nodeSphere(L, meshes, numMeshes, meshIndex[i]);
 		return;
 	}
-	float4 W[4];
+	// the parent index and the local block in flight together, then the
+	// parent matrix while the block goes through LDS
 	const uint32_t p = live ? parentIndex[i] : ~0u;
-	if (p < parentCount) {
-		float4 P[4];
+	float4 raw[4];
+	issueMatrixLoads(local, first, count, lane, raw);
+	const bool hasParent = p < parentCount;
+	float4 P[4];
+	if (hasParent) {
 #pragma unroll
 		for (int c = 0; c < 4; c++) P[c] = parentWorld[4 * uint64_t(p) + c];
+	}
+	stageMatrices(stage[wave], lane, raw, L);
+	float4 W[4];
+	if (hasParent) {
 		mat4Mul(P, L, W);
 	} else {
 #pragma unroll
@@ -104,13 +112,20 @@ __device__ __forceinline__ void propagateUnit(const cadrTransformLevel& L, uint6
 	const uint64_t i = first + lane;
 	const bool live = i < count;
 	float4 M[4], W[4];
-	loadMatrices(reinterpret_cast<const GlobalF4*>(L.localMatrices), first, count, stage, lane, M);
+	// the parent index and the local block in flight together, then the
+	// parent matrix while the block goes through LDS
 	const uint32_t p = (L.parentWorld && live) ? reinterpret_cast<GlobalU32*>(L.parentIndex)[i] : ~0u;
-	if (p < L.parentCount) {
+	float4 raw[4];
+	issueMatrixLoads(reinterpret_cast<const GlobalF4*>(L.localMatrices), first, count, lane, raw);
+	const bool hasParent = p < L.parentCount;
+	float4 P[4];
+	if (hasParent) {
 		const GlobalF4* pw = reinterpret_cast<const GlobalF4*>(L.parentWorld);
-		float4 P[4];
 #pragma unroll
 		for (int c = 0; c < 4; c++) P[c] = loadF4(pw + (4 * uint64_t(p) + c));
+	}
+	stageMatrices(stage, lane, raw, M);
+	if (hasParent) {
 		mat4Mul(P, M, W);
 	} else {
 #pragma unroll
