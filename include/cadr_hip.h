@@ -97,7 +97,10 @@ int cadrEventElapsedTime(float* milliseconds, cadrEvent start, cadrEvent stop);
  * cadrEventRecord is a command of its own and adds a few microseconds to a
  * chain of kernels).  An entry point that launches nothing records both back
  * to back; one that fails before launching leaves them pending.  The events
- * stay pending until such a call; cadrTimeNextLaunch(NULL, NULL) clears them.
+ * stay pending until such a call, also one made inside another entry point
+ * on this thread (cadrCullAutotune's first candidate, the facade's frames);
+ * cadrTimeNextLaunch(NULL, NULL) clears them.  Not for use while the stream
+ * is being captured into a HIP graph.
  * Plays the part of the reference's vkCmdWriteTimestamp pair around its
  * compute dispatch (src/CadR/Renderer.cpp:660-700; both written at once
  * when there is nothing to dispatch, :600-617). */
