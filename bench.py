@@ -541,7 +541,7 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device, p
     log(f"batch{V}: {batch_ms:.3f} ms/frame ({tests / (batch_ms * 1e-3) / 1e9:.1f} G tests/s, {gbs:.0f} GB/s alg), "
         f"plain {plain_ms} ms, identical={same}")
     res["frame"] = batched_frames(args, [(ds, views, out)], [leaf_desc], propagate_inner, batch, stream, lib, N, cull,
-                                  f"batch{V}")
+                                  f"batch{V}", inner_nodes=int(sum(lv[0] for lv in ds.levels[:-1])))
     if not args.no_cpu:
         ds.propagate(s)
         stream.synchronize()
@@ -556,13 +556,14 @@ def bench_batched(args, ds, view_config, stream, lib, N, cull, scenes, device, p
     return res, config
 
 
-def batched_frames(args, entries, leaf_descs, propagate_inner, batch, stream, lib, N, cull, name, unfused=None):
+def batched_frames(args, entries, leaf_descs, propagate_inner, batch, stream, lib, N, cull, name, unfused=None,
+                   inner_nodes=0):
     """Whole frames of a batched workload, each one HIP graph: the levels
     above the leaves (propagate_inner), then either every job's leaf level as
     its own launch (world spheres written) followed by the batched cull (or
     the given unfused frame), or the leaf levels fused into the batched launch
     (cadrPropagateCullBatch, no spheres in HBM), for each fused shape; counts
-    of both compared."""
+    of both compared.  inner_nodes: nodes of the levels above the leaves."""
     s = stream.handle
     steps = max(3, args.batch_steps)
     leaves = np.concatenate(leaf_descs)
@@ -603,7 +604,14 @@ def batched_frames(args, entries, leaf_descs, propagate_inner, batch, stream, li
     fused_bytes = (leaves_n * (64 + 4 + 4 + 64) + parents_n * 64 + survivors * 20 +
                    sum(e[0].K * e[1].shape[0] * 4 for e in entries))
     fused_gbs = fused_bytes / (fused_ms * 1e-3) / 1e9
-    r = {"unfused_ms_per_frame": un_ms, "fused_ms_per_frame": sweep[best], "fused_shape": best,
+    # the unfused frame: 64 + 4 B read and 64 B written per node above the
+    # leaves, 152 B per leaf (its world sphere written too) + 64 B per parent,
+    # then the batched cull: 20 B per drawable read once, 20 B per visible
+    # command and 4 B per (view, StateSet) count written
+    unfused_bytes = (inner_nodes * (64 + 4 + 64) + leaves_n * (64 + 4 + 4 + 64 + 16) + parents_n * 64 +
+                     leaves_n * 20 + survivors * 20 + sum(e[0].K * e[1].shape[0] * 4 for e in entries))
+    r = {"unfused_ms_per_frame": un_ms, "unfused_algorithmic_bytes": unfused_bytes,
+         "unfused_hbm_frac": unfused_bytes / (un_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "fused_ms_per_frame": sweep[best], "fused_shape": best,
          "fused_candidates_ms": sweep, "speedup_fused": un_ms / sweep[best], "fused_counts_identical": bool(same),
          "fused_roofline": {"bound": "hbm", "kernel": f"cullKernel<{best.replace('x', ',')},true>",
                             "kernel_ms": fused_ms, "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -757,7 +765,8 @@ def bench_uneven(args, view_config, stream, lib, N, cull, scenes, device):
     log(f"uneven64: frame {frame_ms:.3f} ms, cull {cull_ms:.3f} ms ({cull_gbs:.0f} GB/s alg), "
         f"plain cull {plain_ms} ms, identical={same}")
     res["fused_frame"] = batched_frames(args, entries, [lvs[-1] for lvs in levels], propagate_inner, batch, stream,
-                                        lib, N, cull, "uneven64", unfused=lambda: (propagate_all(), batch.run(s)))
+                                        lib, N, cull, "uneven64", unfused=lambda: (propagate_all(), batch.run(s)),
+                                        inner_nodes=nodes - leaves)
     res["fused_frame"]["culled_drawables_per_s"] = tests / (res["fused_frame"]["fused_ms_per_frame"] * 1e-3)
     if not args.no_cpu:
         # the first 8 scenes' inputs (generated on the device) read back

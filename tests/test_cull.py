@@ -115,6 +115,28 @@ def test_cull_one_view_large_workgroups(plain, block, items):
         _run(ds, views, cull.make_config(block, items))
 
 
+def test_cull_one_view_persistent_loop(gpu):
+    """A view with more tiles than workgroups fit on the GPU runs the
+    persistent one-view kernel (next tile's loads in flight during the
+    current tile's look-back): 2M drawables in 8 StateSets, shapes whose
+    tile count exceeds their resident workgroups, against the oracle, for
+    the default camera and a random one."""
+    scene = scenes.plain_scene(2_000_000, seed=scenes.SEED_BASE + 21, num_statesets=8)
+    ds = cull.DeviceScene(scene)
+    ds.propagate()
+    spheres = ds.download_spheres()
+    cams = [scenes.DEFAULT_CAMERA] + scenes.random_cameras(scenes.SEED_BASE + 21, 20, 0, 1, box=400.0)
+    for c in cams:
+        views = cull.make_views([c])
+        exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
+                                           cmd_dtype=N.CMD_DTYPE, nthreads=8)
+        for block, items in [(256, 2), (256, 8), (512, 4), (1024, 1), (1024, 4)]:
+            print(f"plain 2M view block={block} items={items} visible={int(exp_counts.sum())}", flush=True)
+            cmds, counts = _run(ds, views, cull.make_config(block, items), single=True)
+            _compare(cmds, counts, exp_cmds, exp_counts, scene.stateset_counts, f"plain 2M {block}x{items}")
+    ds.close()
+
+
 def test_cull_views_match_oracle_views(plain):
     cams = scenes.batch_cameras(64)
     for c in cams:
