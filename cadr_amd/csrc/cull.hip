@@ -19,11 +19,15 @@
 // over StateSets, so the walks stay short.  Status words are tagged with a
 // launch epoch and never cleared.
 //
-// Three kernels:
+// Four kernels:
 //   cullViewKernel - one view: one workgroup per tile, visibility kept in
 //     registers, a 64-lane parallel look-back in wave 0, each visible
-//     drawable's lane storing its command at its compacted slot.  It resets its ticket and advances the
-//     epoch itself (the last workgroup to finish), so a view costs one launch.
+//     drawable's lane storing its command at its compacted slot.  It resets
+//     its ticket and advances the epoch itself (the last workgroup to
+//     finish), so a view costs one launch.
+//   cullViewLoopKernel - the same view pass as a persistent loop over tiles,
+//     the next tile's loads in flight during the current tile's look-back
+//     and stores; used when a view has more tiles than resident workgroups.
 //   cullFusedKernel - the same for one view, with the tile's spheres computed
 //     from the last transform level instead of read (cadrPropagateCullView).
 //   cullKernel - many views and many scenes (jobs) in one persistent launch:
