@@ -109,6 +109,24 @@ __device__ __forceinline__ void stageMatrices(float4* stage, uint32_t lane, cons
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The parent world matrices of a wave's 64 nodes (lane l's parent index p),
+// read whole: lane l loads column l % 4 of the parent of lane 16k + l / 4 for
+// k = 0..3, so each instruction reads 16 parents (one or two 128-byte lines
+// each) instead of one column of 64 parents scattered over 64 lines; the
+// block then goes through LDS (stageMatrices), which hands lane l its own
+// parent.  Indices at or past parentCount read parent 0 (parentCount >= 1),
+// and the caller ignores those lanes' matrices.
+template <class Src>
+__device__ __forceinline__ void issueParentLoads(Src parentWorld, uint32_t parentCount, uint32_t p, uint32_t lane,
+                                                 float4 raw[4])
+{
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		const uint32_t q = uint32_t(__shfl(int(p), int(16u * uint32_t(k) + (lane >> 2))));
+		raw[k] = loadF4(parentWorld + (4 * uint64_t(q < parentCount ? q : 0u) + (lane & 3u)));
+	}
+}
+
 template <class Src>
 __device__ __forceinline__ void loadMatrices(Src src, uint64_t first, uint64_t count, float4* stage, uint32_t lane,
                                              float4 M[4])

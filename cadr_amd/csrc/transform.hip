@@ -72,17 +72,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void p
 		return;
 	}
 	// the parent index and the local block in flight together, then the
-	// parent matrix while the block goes through LDS
+	// parent matrices while the block goes through LDS
 	const uint32_t p = live ? parentIndex[i] : ~0u;
 	float4 raw[4];
 	issueMatrixLoads(local, first, count, lane, raw);
 	const bool hasParent = p < parentCount;
-	float4 P[4];
-	if (hasParent) {
-#pragma unroll
-		for (int c = 0; c < 4; c++) P[c] = parentWorld[4 * uint64_t(p) + c];
-	}
+	float4 P[4], praw[4];
+	if (parentCount > 0) issueParentLoads(parentWorld, parentCount, p, lane, praw);
 	stageMatrices(stage[wave], lane, raw, L);
+	if (parentCount > 0) stageMatrices(stage[wave], lane, praw, P);  // the same LDS, after the local block
 	float4 W[4];
 	if (hasParent) {
 		mat4Mul(P, L, W);
@@ -118,13 +116,12 @@ __device__ __forceinline__ void propagateUnit(const cadrTransformLevel& L, uint6
 	float4 raw[4];
 	issueMatrixLoads(reinterpret_cast<const GlobalF4*>(L.localMatrices), first, count, lane, raw);
 	const bool hasParent = p < L.parentCount;
-	float4 P[4];
-	if (hasParent) {
-		const GlobalF4* pw = reinterpret_cast<const GlobalF4*>(L.parentWorld);
-#pragma unroll
-		for (int c = 0; c < 4; c++) P[c] = loadF4(pw + (4 * uint64_t(p) + c));
-	}
+	const bool anyParent = L.parentWorld && L.parentCount > 0;  // the same in every lane
+	float4 P[4], praw[4];
+	if (anyParent)
+		issueParentLoads(reinterpret_cast<const GlobalF4*>(L.parentWorld), L.parentCount, p, lane, praw);
 	stageMatrices(stage, lane, raw, M);
+	if (anyParent) stageMatrices(stage, lane, praw, P);  // the same LDS, after the local block
 	if (hasParent) {
 		mat4Mul(P, M, W);
 	} else {
