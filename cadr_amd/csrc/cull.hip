@@ -553,31 +553,38 @@ __device__ __forceinline__ bool stageMeshes(uint32_t* sMeshes, const cadrCullJob
 // per SIMD for the whole view loop (measured slower).
 struct ItemLoads {
 	float4 raw[4];  // this lane's 4 float4 of the wave's 4-KiB local block
-	float4 P[4];    // this lane's parent world matrix
+	float4 P[4];    // this lane's 4 float4 of the wave's parent block (issueParentLoads)
 };
 
+// Every load unconditional (indices clamped), so all are in flight at once;
+// lanes past the tile end load the tile's last node again and nothing of
+// theirs is stored.
+template <bool COOP>
 __device__ __forceinline__ void issueItemLoads(const LeafParams& Lf, uint64_t first, uint64_t end, uint32_t p,
-                                               bool live, uint32_t lane, ItemLoads& d)
+                                               uint32_t lane, ItemLoads& d)
 {
 	const gptr<float4> local = globalPtr<float4>(Lf.local);
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
 		const uint32_t q = uint32_t(k) * 64u + lane;  // float4 index inside the block
-		d.raw[k] = first + q / 4u < end ? loadF4Stream(local + (4 * first + q)) : make_float4(0.f, 0.f, 0.f, 0.f);
+		const uint64_t m = min(first + q / 4u, end - 1u);
+		d.raw[k] = loadF4Stream(local + (4 * m + (q & 3u)));
 	}
-	if (live && p < Lf.parentCount) {
-		const gptr<float4> parentWorld = globalPtr<float4>(Lf.parentWorld);
-#pragma unroll
-		for (int c = 0; c < 4; c++) d.P[c] = loadF4(parentWorld + (4 * uint64_t(p) + c));
+	if constexpr (COOP) {
+		if (Lf.parentCount > 0) issueParentLoads(globalPtr<float4>(Lf.parentWorld), Lf.parentCount, p, lane, d.P);
 	} else {
+		const uint32_t pc = p < Lf.parentCount ? p : 0u;
+		if (Lf.parentCount > 0) {
+			const gptr<float4> parentWorld = globalPtr<float4>(Lf.parentWorld);
 #pragma unroll
-		for (int c = 0; c < 4; c++) d.P[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+			for (int c = 0; c < 4; c++) d.P[c] = loadF4(parentWorld + (4 * uint64_t(pc) + c));
+		}
 	}
 }
 
 // store: write the world matrices (and spheres); a tile that several work
 // items of one launch share is written by one of them.
-template <int ITEMS, bool LDS, bool PREFETCH = true>
+template <int ITEMS, bool LDS, bool PREFETCH = true, bool COOP = false>
 __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const cadrCullJob& J, const TileDesc& T,
                                                    uint32_t base, uint32_t lane, float4* stage,
                                                    const MeshTable& meshes, float4 (&sp)[ITEMS],
@@ -608,8 +615,7 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 	// items whose 64 nodes all lie past the tile end come last and are empty
 	const uint32_t items = base >= T.count ? 0u : min(uint32_t(ITEMS), (T.count - base + 63u) / 64u);
 	ItemLoads next;
-	if (PREFETCH && items > 0)
-		issueItemLoads(Lf, uint64_t(T.start) + base, end, pi[0], uint64_t(T.start) + base + lane < end, lane, next);
+	if (PREFETCH && items > 0) issueItemLoads<COOP>(Lf, uint64_t(T.start) + base, end, pi[0], lane, next);
 #pragma unroll
 	for (int j = 0; j < ITEMS; j++) {
 		if (uint32_t(j) >= items) {
@@ -623,10 +629,9 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 		if constexpr (PREFETCH) {
 			cur = next;
 			if (uint32_t(j) + 1u < items)
-				issueItemLoads(Lf, first + 64u, end, pi[j + 1 < ITEMS ? j + 1 : j], first + 64u + lane < end, lane,
-				               next);
+				issueItemLoads<COOP>(Lf, first + 64u, end, pi[j + 1 < ITEMS ? j + 1 : j], lane, next);
 		} else {
-			issueItemLoads(Lf, first, end, pi[j], live, lane, cur);
+			issueItemLoads<COOP>(Lf, first, end, pi[j], lane, cur);
 		}
 		// redistribute the local block: lane l takes matrix l
 #pragma unroll
@@ -634,14 +639,21 @@ __device__ __forceinline__ void propagateTileItems(const LeafParams& Lf, const c
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-		float4 L[4], W[4];
+		float4 L[4], W[4], P[4];
 #pragma unroll
 		for (int c = 0; c < 4; c++) L[c] = stage[stageSlot(4 * lane + c)];
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		// the parent block the same way: lane l gets its own parent
+		if constexpr (COOP) {
+			if (Lf.parentCount > 0) stageMatrices(stage, lane, cur.P, P);
+		} else {
+#pragma unroll
+			for (int c = 0; c < 4; c++) P[c] = cur.P[c];
+		}
 		if (live && pi[j] < Lf.parentCount) {
-			mat4Mul(cur.P, L, W);
+			mat4Mul(P, L, W);
 		} else {
 #pragma unroll
 			for (int c = 0; c < 4; c++) W[c] = L[c];
@@ -952,7 +964,10 @@ struct CullShared {
 	uint32_t ticket;
 };
 
-template <int BLOCK, int ITEMS, bool FUSED>
+// COOP (fused form only): parent matrices read whole, 16 per instruction
+// (issueParentLoads), for leaves whose parents are scattered; it holds more
+// registers, so it is chosen per shape (see the kernel table).
+template <int BLOCK, int ITEMS, bool FUSED, bool COOP = false>
 __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 {
 	using S = CullShared<BLOCK, ITEMS>;
@@ -1021,11 +1036,11 @@ __global__ __launch_bounds__(BLOCK) void cullKernel(CullParams P)
 			constexpr bool PREFETCH = ITEMS >= 8;
 			float4* stage = &sMat[wave][0];
 			if (ldsMeshes)
-				propagateTileItems<ITEMS, true, PREFETCH>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp, mi,
-				                                          storeLeaf);
+				propagateTileItems<ITEMS, true, PREFETCH, COOP>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp,
+				                                                mi, storeLeaf);
 			else
-				propagateTileItems<ITEMS, false, PREFETCH>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp, mi,
-				                                           storeLeaf);
+				propagateTileItems<ITEMS, false, PREFETCH, COOP>(Lf, J, T, wave * WAVE_ITEMS, lane, stage, meshes, sp,
+				                                                 mi, storeLeaf);
 		} else {
 			loadTileItems<ITEMS>(globalPtr<float4>(J.spheres), globalPtr<uint32_t>(J.meshIndex), T,
 			                     wave * WAVE_ITEMS + lane, sp, mi);
@@ -1378,14 +1393,17 @@ struct KernelEntry {
 };
 
 // viewLoop where it measured faster at 10M drawables (one view of the full10m
-// scene): not at 512 x 2/8/16, and 1024 x 16's loop form spills registers
+// scene): not at 512 x 2/8/16, and 1024 x 16's loop form spills registers.
+// fusedBatch reads parents whole (COOP) at 256 x 8 and 512 x 4, where it
+// measured faster on uneven64 (2.39 -> 2.32 and 2.93 -> 2.84 ms); at 256 x 4
+// its registers cost a wave per SIMD (2.38 -> 2.60 ms)
 const KernelEntry kKernels[] = {
     {256, 2, cullKernel<256, 2, false>, cullViewKernel<256, 2>, cullFusedKernel<256, 2, 4>, nullptr, cullViewLoopKernel<256, 2>},
     {256, 4, cullKernel<256, 4, false>, cullViewKernel<256, 4>, cullFusedKernel<256, 4, 4>, cullKernel<256, 4, true>, cullViewLoopKernel<256, 4>},
-    {256, 8, cullKernel<256, 8, false>, cullViewKernel<256, 8>, cullFusedKernel<256, 8, 1>, cullKernel<256, 8, true>, cullViewLoopKernel<256, 8>},
+    {256, 8, cullKernel<256, 8, false>, cullViewKernel<256, 8>, cullFusedKernel<256, 8, 1>, cullKernel<256, 8, true, true>, cullViewLoopKernel<256, 8>},
     {256, 16, cullKernel<256, 16, false>, cullViewKernel<256, 16>, cullFusedKernel<256, 16, 1>, nullptr, cullViewLoopKernel<256, 16>},
     {512, 2, cullKernel<512, 2, false>, cullViewKernel<512, 2>, cullFusedKernel<512, 2, 1>, nullptr, nullptr},
-    {512, 4, cullKernel<512, 4, false>, cullViewKernel<512, 4>, cullFusedKernel<512, 4, 1>, cullKernel<512, 4, true>, cullViewLoopKernel<512, 4>},
+    {512, 4, cullKernel<512, 4, false>, cullViewKernel<512, 4>, cullFusedKernel<512, 4, 1>, cullKernel<512, 4, true, true>, cullViewLoopKernel<512, 4>},
     {512, 8, nullptr, cullViewKernel<512, 8>, nullptr, nullptr, nullptr},
     {512, 16, nullptr, cullViewKernel<512, 16>, nullptr, nullptr, nullptr},
     {1024, 1, nullptr, cullViewKernel<1024, 1>, cullFusedKernel<1024, 1, 1>, nullptr, cullViewLoopKernel<1024, 1>},
