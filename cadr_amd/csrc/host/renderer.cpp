@@ -478,8 +478,7 @@ void Renderer::releaseCull() noexcept
 		*bc.first = 0;
 	}
 	for (uint64_t* b : {&_cull.spheres, &_cull.meshIndex, &_cull.itemBase, &_cull.itemSource, &_cull.rangeBase,
-	                    &_cull.tiles, &_cull.views, &_cull.job, &_cull.commands, &_cull.counts, &_cull.pointers,
-	                    &_cull.sources, &_cull.meshes})
+	                    &_cull.tiles, &_cull.upload, &_cull.commands, &_cull.counts, &_cull.pointers, &_cull.meshes})
 		if (*b) (void)hipFree(reinterpret_cast<void*>(*b));
 	_cull = CullBuffers{};
 	if (_cullCopied) (void)hipEventDestroy(hipEvent_t(_cullCopied));
@@ -582,7 +581,9 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 	}
 	if (_cullCopyPending) hipCheck(hipEventSynchronize(hipEvent_t(_cullCopied)), "hipEventSynchronize");
 	_cullCopyPending = false;
-	// pinned staging: job, views, mesh sources (pageable copies would wait for the stream)
+	// pinned staging: job, views, mesh sources, sent as one copy into one
+	// device region (pageable copies would wait for the stream; three small
+	// copies cost a blit launch each)
 	const size_t M = _meshKeys.size();
 	const size_t viewsAt = 128, sourcesAt = viewsAt + (V * sizeof(cadrView) + 127) / 128 * 128;
 	const size_t pinnedBytes = sourcesAt + std::max<size_t>(M, 1) * sizeof(cadrMeshSource);
@@ -616,33 +617,11 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 		const BoundingSphere& b = g->boundingSphere();
 		o.sphere[0] = b.center[0]; o.sphere[1] = b.center[1]; o.sphere[2] = b.center[2]; o.sphere[3] = b.radius;
 	}
-	growCull(_cull.sources, _cull.sourcesCap, std::max<size_t>(M, 1) * sizeof(cadrMeshSource), "hipMalloc(mesh sources)");
+	std::memcpy(viewsHost, views, V * sizeof(cadrView));
+	growCull(_cull.upload, _cull.uploadCap, pinnedBytes, "hipMalloc(cull job, views, mesh sources)");
 	growCull(_cull.meshes, _cull.meshesCap, std::max<size_t>(M, 1) * sizeof(cadrMesh), "hipMalloc(mesh table)");
-	hipCheck(hipMemcpyAsync(reinterpret_cast<void*>(_cull.sources), src, M * sizeof(cadrMeshSource),
-	                        hipMemcpyHostToDevice, s),
-	         "hipMemcpyAsync(mesh sources)");
-	const uint64_t table = _dataStorage->handleTableDeviceAddress();
-	const uint32_t level = _dataStorage->handleLevel();
-	throwOnError(cadrBuildMeshTable(table, level, _cull.sources, uint32_t(M), _cull.meshes, stream), fn);
 
-	// cull items from the resident drawable list
-	if (items > 0) {
-		cadrCullItemsDesc d{};
-		d.handleTable = table;
-		d.drawableList = _drawableBuffer;
-		d.itemBase = _cullIdentity ? 0 : _cull.itemBase;
-		d.itemDrawable = _cullIdentity ? 0 : _cull.itemSource;
-		d.meshes = _cull.meshes;
-		d.spheres = _cull.spheres;
-		d.meshIndex = _cull.meshIndex;
-		d.numDrawables = N;
-		d.numItems = items;
-		d.handleLevel = level;
-		d.numMeshes = uint32_t(M);
-		throwOnError(cadrBuildCullItems(&d, stream), fn);
-	}
-
-	// outputs, views, tiles and the job
+	// outputs, tiles and the workspace
 	if (_cullShared) {
 		growCullShared(_cull.commands, _cull.commandsCap, _cull.sharedCommands, std::max<size_t>(V * items * 20, 4),
 		               "cadrSharedBufferCreate(cull commands)");
@@ -657,11 +636,6 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 		if (_cullPointersOn)
 			growCull(_cull.pointers, _cull.pointersCap, std::max<size_t>(V * items * 32, 16), "hipMalloc(cull pointers)");
 	}
-	growCull(_cull.views, _cull.viewsCap, V * sizeof(cadrView), "hipMalloc(cull views)");
-	std::memcpy(viewsHost, views, V * sizeof(cadrView));
-	hipCheck(hipMemcpyAsync(reinterpret_cast<void*>(_cull.views), viewsHost, V * sizeof(cadrView),
-	                        hipMemcpyHostToDevice, s),
-	         "hipMemcpyAsync(cull views)");
 	if (_cullTileSize != tile) {
 		std::vector<uint32_t> counts(K);
 		for (size_t k = 0; k < K; k++) counts[k] = uint32_t(_cullRanges[k].itemCount);
@@ -691,7 +665,7 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 	job.meshIndex = _cull.meshIndex;
 	job.meshes = _cull.meshes;
 	job.tiles = _cull.tiles;
-	job.views = _cull.views;
+	job.views = _cull.upload + viewsAt;
 	job.commands = _cull.commands;
 	job.counts = _cull.counts;
 	job.numDrawables = uint32_t(items);
@@ -700,15 +674,36 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 	job.numViews = uint32_t(V);
 	job.commandStride = uint32_t(items);
 	job.numMeshes = uint32_t(M);
-	growCull(_cull.job, _cull.jobCap, sizeof(cadrCullJob), "hipMalloc(cull job)");
-	hipCheck(hipMemcpyAsync(reinterpret_cast<void*>(_cull.job), &job, sizeof(job), hipMemcpyHostToDevice, s),
-	         "hipMemcpyAsync(cull job)");
+	hipCheck(hipMemcpyAsync(reinterpret_cast<void*>(_cull.upload), _cullPinned, pinnedBytes, hipMemcpyHostToDevice, s),
+	         "hipMemcpyAsync(cull job, views, mesh sources)");
 	hipCheck(hipEventRecord(hipEvent_t(_cullCopied), s), "hipEventRecord");
 	_cullCopyPending = true;
+
+	const uint64_t table = _dataStorage->handleTableDeviceAddress();
+	const uint32_t level = _dataStorage->handleLevel();
+	throwOnError(cadrBuildMeshTable(table, level, _cull.upload + sourcesAt, uint32_t(M), _cull.meshes, stream), fn);
+
+	// cull items from the resident drawable list
+	if (items > 0) {
+		cadrCullItemsDesc d{};
+		d.handleTable = table;
+		d.drawableList = _drawableBuffer;
+		d.itemBase = _cullIdentity ? 0 : _cull.itemBase;
+		d.itemDrawable = _cullIdentity ? 0 : _cull.itemSource;
+		d.meshes = _cull.meshes;
+		d.spheres = _cull.spheres;
+		d.meshIndex = _cull.meshIndex;
+		d.numDrawables = N;
+		d.numItems = items;
+		d.handleLevel = level;
+		d.numMeshes = uint32_t(M);
+		throwOnError(cadrBuildCullItems(&d, stream), fn);
+	}
+
 	if (V == 1)
-		throwOnError(cadrCullView(&cfg, _cull.job, &job, _cullWs, stream), fn);
+		throwOnError(cadrCullView(&cfg, _cull.upload, &job, _cullWs, stream), fn);
 	else
-		throwOnError(cadrCullBatch(&cfg, _cull.job, &job, 1, _cullWs, stream), fn);
+		throwOnError(cadrCullBatch(&cfg, _cull.upload, &job, 1, _cullWs, stream), fn);
 
 	// the reference's draw contract: instance index and DrawablePointers per command
 	throwOnError(cadrFinishCullCommands(_cull.commands, _cull.counts, _cull.rangeBase, uint32_t(V), uint32_t(K),

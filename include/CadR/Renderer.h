@@ -246,9 +246,10 @@ private:
 	cadrCullConfig _cullSingle{1024, 8, 0, 0}, _cullBatched{256, 8, 0, 0};
 	struct CullBuffers {
 		uint64_t spheres = 0, meshIndex = 0, itemBase = 0, itemSource = 0, rangeBase = 0, tiles = 0;
-		uint64_t views = 0, job = 0, commands = 0, counts = 0, pointers = 0, sources = 0, meshes = 0;
+		uint64_t upload = 0;  // a frame's job, views and mesh sources, one copy
+		uint64_t commands = 0, counts = 0, pointers = 0, meshes = 0;
 		size_t spheresCap = 0, meshIndexCap = 0, itemBaseCap = 0, itemSourceCap = 0, rangeBaseCap = 0, tilesCap = 0;
-		size_t viewsCap = 0, jobCap = 0, commandsCap = 0, countsCap = 0, pointersCap = 0, sourcesCap = 0, meshesCap = 0;
+		size_t uploadCap = 0, commandsCap = 0, countsCap = 0, pointersCap = 0, meshesCap = 0;
 		cadrSharedBuffer *sharedCommands = nullptr, *sharedCounts = nullptr, *sharedPointers = nullptr;
 	} _cull;
 	bool _cullShared = false;
