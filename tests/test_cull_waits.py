@@ -91,3 +91,53 @@ def test_missing_predecessor_times_out_and_workspace_recovers(small, path):
         b.close()
     out.close()
     ws.close()
+
+
+def test_missing_predecessor_in_the_persistent_view_kernel(gpu):
+    """The same through the persistent one-view kernel, which runs when a
+    view has more tiles than resident workgroups (800k drawables in tiles
+    of 1024; two 1024-thread workgroups fit per CU): a list without its first
+    tile ends after the 1-ms limit and is reported; the full list then gives
+    the oracle's draw list on the same workspace."""
+    import ctypes
+    scene = scenes.plain_scene(800_000, seed=scenes.SEED_BASE + 23)
+    ds = cull.DeviceScene(scene)
+    ds.propagate()
+    spheres = ds.download_spheres()
+    config = cull.make_config(1024, 1)
+    good = cull.build_tiles(scene.stateset_counts, cull.tile_size(config))
+    assert good.shape[0] > 2 * 256
+    bad = good[1:].copy()
+    print(f"missing-predecessor tile list, persistent view kernel, {bad.shape[0]} tiles", flush=True)
+    views = cull.make_views([scenes.DEFAULT_CAMERA])
+    vb = DeviceBuffer.from_array(views)
+    good_b, bad_b = DeviceBuffer.from_array(good), DeviceBuffer.from_array(bad)
+    out = cull.CullOutputs(1, ds.n, ds.K)
+    ws = cull.Workspace(good.shape[0], 1)
+    ws.set_wait_limit(1000)
+    lib = N.load()
+
+    def launch(tb, nt):
+        jobs = _job(ds, tb, nt, vb, 1, out)
+        jb = DeviceBuffer.from_array(jobs)
+        N.check(lib.cadrCullView(ctypes.byref(config), jb.address, N.ptr(jobs), ws.handle, None))
+        synchronize()
+        jb.close()
+
+    launch(bad_b, bad.shape[0])
+    with pytest.raises(N.CadrError) as e:
+        ws.check()
+    assert e.value.code == N.ERROR_TIMEOUT
+    launch(good_b, good.shape[0])
+    ws.check()
+    cmds, counts = out.download()
+    exp_cmds, exp_counts = oracle.cull(spheres, scene.mesh_index, scene.meshes, scene.stateset_counts, views,
+                                       cmd_dtype=N.CMD_DTYPE)
+    assert np.array_equal(counts, exp_counts)
+    n = int(exp_counts[0, 0])
+    assert n > 0 and cmds[0, :n].tobytes() == exp_cmds[0, :n].tobytes()
+    for b in (vb, good_b, bad_b):
+        b.close()
+    out.close()
+    ws.close()
+    ds.close()
