@@ -553,7 +553,8 @@ __device__ __forceinline__ bool stageMeshes(uint32_t* sMeshes, const cadrCullJob
 // per SIMD for the whole view loop (measured slower).
 struct ItemLoads {
 	float4 raw[4];  // this lane's 4 float4 of the wave's 4-KiB local block
-	float4 P[4];    // this lane's 4 float4 of the wave's parent block (issueParentLoads)
+	float4 P[4];    // this lane's parent world matrix, or with COOP its 4 float4 of the
+	                // wave's parent block (issueParentLoads), staged through LDS before use
 };
 
 // Every load unconditional (indices clamped), so all are in flight at once;
