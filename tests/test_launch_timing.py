@@ -110,3 +110,57 @@ def test_timed_cull_counts_match_untimed(gpu):
     start.close()
     stop.close()
     stream.close()
+
+
+def test_timed_batched_and_fused_launches_match_untimed(gpu):
+    """cadrCullBatch, cadrPropagateCullView and cadrPropagateCullBatch with
+    dispatch-bound events: each timed launch gives the counts of the same
+    launch untimed, and its interval is positive."""
+    spec = scenes.TreeSpec((20, 50, 100), seed=scenes.SEED_BASE + 93, name="timed3")
+    stream = Stream()
+    s = stream.handle
+    ds = cull.DeviceScene(spec)
+    ds.propagate(s)
+    views = cull.make_views(scenes.batch_cameras(8))
+    one = cull.make_views([scenes.DEFAULT_CAMERA])
+    leaf = ds.level_descs(spheres=True)[-1].copy()
+    leaf["spheres"] = 0
+    cases = [("batch", cull.make_config(256, 8), views, lambda b: b.run(s)),
+             ("fused view", cull.make_config(512, 4), one, lambda b: b.run_fused(leaf, s))]
+    for name, cfg, v, run in cases:
+        outs = [cull.CullOutputs(v.shape[0], ds.n, ds.K) for _ in range(2)]
+        runs = [cull.CullBatch(cfg, [(ds, v, o)]) for o in outs]
+        run(runs[0])
+        start, stop = Event(), Event()
+        time_next_launch(start, stop)
+        run(runs[1])
+        stream.synchronize()
+        assert 0.0 < start.elapsed_ms(stop) < 100.0, name
+        c0, c1 = outs[0].download_counts(), outs[1].download_counts()
+        assert c0.sum() > 0 and np.array_equal(c0, c1), name
+        for r in runs:
+            r.close()
+        for o in outs:
+            o.close()
+        start.close()
+        stop.close()
+    # the fused batched form (leaf levels in device memory)
+    outs = [cull.CullOutputs(views.shape[0], ds.n, ds.K) for _ in range(2)]
+    runs = [cull.CullBatch(cull.make_config(256, 8), [(ds, views, o)]) for o in outs]
+    for r in runs:
+        r.set_leaves(leaf)
+    runs[0].run_fused_batch(s)
+    start, stop = Event(), Event()
+    time_next_launch(start, stop)
+    runs[1].run_fused_batch(s)
+    stream.synchronize()
+    assert 0.0 < start.elapsed_ms(stop) < 100.0
+    c0, c1 = outs[0].download_counts(), outs[1].download_counts()
+    assert c0.sum() > 0 and np.array_equal(c0, c1)
+    for r in runs:
+        r.close()
+    for o in outs:
+        o.close()
+    start.close()
+    stop.close()
+    stream.close()
