@@ -21,8 +21,8 @@ int computeUnitCount();
 // Device memory for the library's large buffers.  From kMappedAllocMinBytes
 // on, the memory is a physical allocation mapped into a reserved address
 // range (hipMemCreate + hipMemMap) instead of hipMalloc: the cull kernels'
-// scattered command stores into tens of GB measured about 10% faster in
-// such memory.  Smaller sizes, or a failure on that path, use hipMalloc.
+// scattered command stores into tens of GB measured 2-10% faster in such
+// memory.  Smaller sizes, or a failure on that path, use hipMalloc.
 // deviceFree releases either kind; like hipFree it waits for the device
 // first.
 constexpr uint64_t kMappedAllocMinBytes = 64ull << 20;

@@ -67,7 +67,7 @@ int cadrGetDeviceCount(int* count);
 int cadrGetComputeUnitCount(int* count);
 /* Device memory.  From 64 MiB on it is a physical allocation mapped into a
  * reserved address range (hipMemCreate + hipMemMap; the cull's scattered
- * command stores ran about 2% faster there than in hipMalloc memory),
+ * command stores ran 2-10% faster there than in hipMalloc memory),
  * below that, or when that path fails, hipMalloc memory.  Either is
  * ordinary device memory for copies and kernels; it has no IPC handle.
  * cadrDeviceFree takes only addresses cadrDeviceMalloc returned and, like
