@@ -4,8 +4,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include "../internal.h"
-
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -69,7 +67,7 @@ void Renderer::finalize()
 	_uploadPending = _listFreePending = false;
 	releaseCull();
 	if (_delta) (void)hipHostFree(_delta);
-	if (_deltaDevice) (void)cadr::deviceFree(reinterpret_cast<void*>(_deltaDevice));
+	if (_deltaDevice) (void)hipFree(reinterpret_cast<void*>(_deltaDevice));
 	_delta = nullptr;
 	_deltaDevice = 0;
 	_deltaCapacity = 0;
@@ -80,10 +78,10 @@ void Renderer::finalize()
 void Renderer::releaseBuffers() noexcept
 {
 	if (_device >= 0) {
-		(void)cadr::deviceFree(reinterpret_cast<void*>(_drawableBuffer));
-		(void)cadr::deviceFree(reinterpret_cast<void*>(_drawableBufferSpare));
-		(void)cadr::deviceFree(reinterpret_cast<void*>(_indirectBuffer));
-		(void)cadr::deviceFree(reinterpret_cast<void*>(_pointersBuffer));
+		(void)hipFree(reinterpret_cast<void*>(_drawableBuffer));
+		(void)hipFree(reinterpret_cast<void*>(_drawableBufferSpare));
+		(void)hipFree(reinterpret_cast<void*>(_indirectBuffer));
+		(void)hipFree(reinterpret_cast<void*>(_pointersBuffer));
 		if (_staging) (void)hipHostFree(_staging);
 	} else {
 		MemoryProvider& p = hostMemoryProvider();
@@ -160,9 +158,9 @@ size_t Renderer::prepareSceneRendering(StateSet& root)
 		releaseBuffers();
 		if (_device >= 0) {
 			void *d = nullptr, *i = nullptr, *p = nullptr;
-			hipCheck(cadr::deviceAlloc(&d, slots * sizeof(DrawableGpuData)), "device allocation (drawable buffer)");
-			hipCheck(cadr::deviceAlloc(&i, slots * sizeof(cadrDrawIndirectCommand)), "device allocation (indirect buffer)");
-			hipCheck(cadr::deviceAlloc(&p, slots * drawablePointersRecordSize), "device allocation (drawable pointers buffer)");
+			hipCheck(hipMalloc(&d, slots * sizeof(DrawableGpuData)), "hipMalloc(drawable buffer)");
+			hipCheck(hipMalloc(&i, slots * sizeof(cadrDrawIndirectCommand)), "hipMalloc(indirect buffer)");
+			hipCheck(hipMalloc(&p, slots * drawablePointersRecordSize), "hipMalloc(drawable pointers buffer)");
 			hipCheck(hipHostMalloc(reinterpret_cast<void**>(&_staging), slots * sizeof(DrawableGpuData),
 			                       hipHostMallocDefault),
 			         "hipHostMalloc(drawable staging)");
@@ -263,14 +261,14 @@ void Renderer::reserveDelta(size_t count)
 	const size_t cap = std::max<size_t>({count, 2 * _deltaCapacity, 1024});
 	const size_t bytes = cap * (sizeof(DrawableGpuData) + sizeof(uint32_t));
 	if (_delta) (void)hipHostFree(_delta);
-	if (_deltaDevice) (void)cadr::deviceFree(reinterpret_cast<void*>(_deltaDevice));
+	if (_deltaDevice) (void)hipFree(reinterpret_cast<void*>(_deltaDevice));
 	_delta = nullptr;
 	_deltaDevice = 0;
 	_deltaCapacity = 0;
 	void *h = nullptr, *d = nullptr;
 	hipCheck(hipHostMalloc(&h, bytes, hipHostMallocDefault), "hipHostMalloc(drawable deltas)");
 	_delta = static_cast<uint8_t*>(h);
-	hipCheck(cadr::deviceAlloc(&d, bytes), "device allocation (drawable deltas)");
+	hipCheck(hipMalloc(&d, bytes), "hipMalloc(drawable deltas)");
 	_deltaDevice = reinterpret_cast<uint64_t>(d);
 	_deltaCapacity = cap;
 }
@@ -292,7 +290,7 @@ void Renderer::uploadResidentList(void* stream, const std::vector<DrawRange>& ra
 	if (!same) {
 		if (!_drawableBufferSpare) {
 			void* p = nullptr;
-			hipCheck(cadr::deviceAlloc(&p, _capacity * R), "device allocation (drawable buffer)");
+			hipCheck(hipMalloc(&p, _capacity * R), "hipMalloc(drawable buffer)");
 			_drawableBufferSpare = reinterpret_cast<uint64_t>(p);
 		}
 		dst = _drawableBufferSpare;
@@ -398,12 +396,12 @@ void Renderer::setCullConfig(const cadrCullConfig& singleView, const cadrCullCon
 void Renderer::growCull(uint64_t& buffer, size_t& capacity, size_t bytes, const char* what)
 {
 	if (bytes <= capacity && buffer) return;
-	(void)cadr::deviceFree(reinterpret_cast<void*>(buffer));
+	(void)hipFree(reinterpret_cast<void*>(buffer));
 	buffer = 0;
 	capacity = 0;
 	const size_t cap = std::max<size_t>(256, bytes + bytes / 5);
 	void* p = nullptr;
-	hipCheck(cadr::deviceAlloc(&p, cap), what);
+	hipCheck(hipMalloc(&p, cap), what);
 	buffer = reinterpret_cast<uint64_t>(p);
 	capacity = cap;
 }
@@ -419,7 +417,7 @@ void Renderer::growCullShared(uint64_t& buffer, size_t& capacity, cadrSharedBuff
 		hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
 		cadrSharedBufferDestroy(shared);
 	} else if (buffer) {
-		(void)cadr::deviceFree(reinterpret_cast<void*>(buffer));
+		(void)hipFree(reinterpret_cast<void*>(buffer));
 	}
 	shared = nullptr;
 	buffer = 0;
@@ -442,7 +440,7 @@ void Renderer::setCullBuffersShared(bool on)
 	                std::make_pair(&_cull.counts, &_cull.sharedCounts),
 	                std::make_pair(&_cull.pointers, &_cull.sharedPointers)}) {
 		if (*bc.second) cadrSharedBufferDestroy(*bc.second);
-		else if (*bc.first) (void)cadr::deviceFree(reinterpret_cast<void*>(*bc.first));
+		else if (*bc.first) (void)hipFree(reinterpret_cast<void*>(*bc.first));
 		*bc.first = 0;
 		*bc.second = nullptr;
 	}
@@ -470,7 +468,7 @@ int Renderer::exportCullPointersBufferFd() const
 
 void Renderer::releaseCull() noexcept
 {
-	// shared output buffers are released as such, the rest with deviceFree
+	// shared output buffers are released as such, the rest with hipFree
 	for (auto bc : {std::make_pair(&_cull.commands, &_cull.sharedCommands),
 	                std::make_pair(&_cull.counts, &_cull.sharedCounts),
 	                std::make_pair(&_cull.pointers, &_cull.sharedPointers)}) {
@@ -481,7 +479,7 @@ void Renderer::releaseCull() noexcept
 	}
 	for (uint64_t* b : {&_cull.spheres, &_cull.meshIndex, &_cull.itemBase, &_cull.itemSource, &_cull.rangeBase,
 	                    &_cull.tiles, &_cull.upload, &_cull.commands, &_cull.counts, &_cull.pointers, &_cull.meshes})
-		if (*b) (void)cadr::deviceFree(reinterpret_cast<void*>(*b));
+		if (*b) (void)hipFree(reinterpret_cast<void*>(*b));
 	_cull = CullBuffers{};
 	if (_cullCopied) (void)hipEventDestroy(hipEvent_t(_cullCopied));
 	_cullCopied = nullptr;
@@ -536,22 +534,22 @@ void Renderer::updateCullLayout(void* stream)
 		if (items >= 0xffffffffull) throw OutOfResources("CadR::Renderer::recordDrawableCulling(): more than 2^32 - 1 cull items");
 	}
 	if (!identity) {
-		growCull(_cull.itemBase, _cull.itemBaseCap, itemBase.size() * 4, "device allocation (cull item bases)");
+		growCull(_cull.itemBase, _cull.itemBaseCap, itemBase.size() * 4, "hipMalloc(cull item bases)");
 		hipCheck(hipMemcpy(reinterpret_cast<void*>(_cull.itemBase), itemBase.data(), itemBase.size() * 4,
 		                   hipMemcpyHostToDevice),
 		         "hipMemcpy(cull item bases)");
-		growCull(_cull.itemSource, _cull.itemSourceCap, std::max<size_t>(items, 1) * 4, "device allocation (cull item drawables)");
+		growCull(_cull.itemSource, _cull.itemSourceCap, std::max<size_t>(items, 1) * 4, "hipMalloc(cull item drawables)");
 		if (items)
 			hipCheck(hipMemcpy(reinterpret_cast<void*>(_cull.itemSource), itemDrawable.data(), items * 4,
 			                   hipMemcpyHostToDevice),
 			         "hipMemcpy(cull item drawables)");
 	}
-	growCull(_cull.rangeBase, _cull.rangeBaseCap, rangeBase.size() * 4, "device allocation (cull range bases)");
+	growCull(_cull.rangeBase, _cull.rangeBaseCap, rangeBase.size() * 4, "hipMalloc(cull range bases)");
 	hipCheck(hipMemcpy(reinterpret_cast<void*>(_cull.rangeBase), rangeBase.data(), rangeBase.size() * 4,
 	                   hipMemcpyHostToDevice),
 	         "hipMemcpy(cull range bases)");
-	growCull(_cull.spheres, _cull.spheresCap, items * 16, "device allocation (cull spheres)");
-	growCull(_cull.meshIndex, _cull.meshIndexCap, items * 4, "device allocation (cull mesh indices)");
+	growCull(_cull.spheres, _cull.spheresCap, items * 16, "hipMalloc(cull spheres)");
+	growCull(_cull.meshIndex, _cull.meshIndexCap, items * 4, "hipMalloc(cull mesh indices)");
 	_cullIdentity = identity;
 	_cullItems = items;
 	_cullRanges = _pendingRanges;
@@ -620,8 +618,8 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 		o.sphere[0] = b.center[0]; o.sphere[1] = b.center[1]; o.sphere[2] = b.center[2]; o.sphere[3] = b.radius;
 	}
 	std::memcpy(viewsHost, views, V * sizeof(cadrView));
-	growCull(_cull.upload, _cull.uploadCap, pinnedBytes, "device allocation (cull job, views, mesh sources)");
-	growCull(_cull.meshes, _cull.meshesCap, std::max<size_t>(M, 1) * sizeof(cadrMesh), "device allocation (mesh table)");
+	growCull(_cull.upload, _cull.uploadCap, pinnedBytes, "hipMalloc(cull job, views, mesh sources)");
+	growCull(_cull.meshes, _cull.meshesCap, std::max<size_t>(M, 1) * sizeof(cadrMesh), "hipMalloc(mesh table)");
 
 	// outputs, tiles and the workspace
 	if (_cullShared) {
@@ -633,10 +631,10 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 			growCullShared(_cull.pointers, _cull.pointersCap, _cull.sharedPointers,
 			               std::max<size_t>(V * items * 32, 16), "cadrSharedBufferCreate(cull pointers)");
 	} else {
-		growCull(_cull.commands, _cull.commandsCap, std::max<size_t>(V * items * 20, 4), "device allocation (cull commands)");
-		growCull(_cull.counts, _cull.countsCap, V * K * 4, "device allocation (cull counts)");
+		growCull(_cull.commands, _cull.commandsCap, std::max<size_t>(V * items * 20, 4), "hipMalloc(cull commands)");
+		growCull(_cull.counts, _cull.countsCap, V * K * 4, "hipMalloc(cull counts)");
 		if (_cullPointersOn)
-			growCull(_cull.pointers, _cull.pointersCap, std::max<size_t>(V * items * 32, 16), "device allocation (cull pointers)");
+			growCull(_cull.pointers, _cull.pointersCap, std::max<size_t>(V * items * 32, 16), "hipMalloc(cull pointers)");
 	}
 	if (_cullTileSize != tile) {
 		std::vector<uint32_t> counts(K);
@@ -645,7 +643,7 @@ void Renderer::recordDrawableCulling(void* stream, const cadrView* views, size_t
 		throwOnError(cadrBuildCullTiles(counts.data(), uint32_t(K), tile, nullptr, 0, &nt), fn);
 		std::vector<cadrCullTile> tiles(nt);
 		throwOnError(cadrBuildCullTiles(counts.data(), uint32_t(K), tile, tiles.data(), nt, &nt), fn);
-		growCull(_cull.tiles, _cull.tilesCap, nt * sizeof(cadrCullTile), "device allocation (cull tiles)");
+		growCull(_cull.tiles, _cull.tilesCap, nt * sizeof(cadrCullTile), "hipMalloc(cull tiles)");
 		hipCheck(hipMemcpy(reinterpret_cast<void*>(_cull.tiles), tiles.data(), nt * sizeof(cadrCullTile),
 		                   hipMemcpyHostToDevice),
 		         "hipMemcpy(cull tiles)");

@@ -3,8 +3,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include "../internal.h"
-
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -36,13 +34,13 @@ struct HipProvider final : MemoryProvider {
 	uint64_t allocate(size_t bytes) override
 	{
 		void* p = nullptr;
-		if (cadr::deviceAlloc(&p, bytes) != hipSuccess) {
+		if (hipMalloc(&p, bytes) != hipSuccess) {
 			(void)hipGetLastError();
 			throw OutOfResources("CadR::DataStorage: cannot allocate " + std::to_string(bytes) + " bytes of HIP memory");
 		}
 		return reinterpret_cast<uint64_t>(p);
 	}
-	void release(uint64_t a) noexcept override { (void)cadr::deviceFree(reinterpret_cast<void*>(a)); }
+	void release(uint64_t a) noexcept override { (void)hipFree(reinterpret_cast<void*>(a)); }
 	void upload(uint64_t dst, const void* src, size_t bytes) override
 	{
 		if (hipMemcpy(reinterpret_cast<void*>(dst), src, bytes, hipMemcpyHostToDevice) != hipSuccess)

@@ -18,17 +18,6 @@ inline int ok() { return CADR_SUCCESS; }
 // Compute-unit count of the current device (cached per device).
 int computeUnitCount();
 
-// Device memory for the library's large buffers.  From kMappedAllocMinBytes
-// on, the memory is a physical allocation mapped into a reserved address
-// range (hipMemCreate + hipMemMap) instead of hipMalloc: the cull kernels'
-// scattered command stores into tens of GB measured 2-10% faster in such
-// memory.  Smaller sizes, or a failure on that path, use hipMalloc.
-// deviceFree releases either kind; like hipFree it waits for the device
-// first.
-constexpr uint64_t kMappedAllocMinBytes = 64ull << 20;
-hipError_t deviceAlloc(void** ptr, uint64_t numBytes);
-hipError_t deviceFree(void* ptr);
-
 // Whether the cull entry point of `path` has a kernel for this shape.
 enum class CullPath { View, Batch, Fused, FusedBatch };
 bool cullHasKernel(const cadrCullConfig* c, CullPath path);

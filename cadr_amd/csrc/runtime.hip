@@ -8,7 +8,6 @@
 #include "launch.h"
 
 #include <mutex>
-#include <unordered_map>
 #include <vector>
 
 namespace cadr {
@@ -60,90 +59,6 @@ int computeUnitCount()
 		cache[dev] = p.multiProcessorCount;
 	}
 	return cache[dev];
-}
-
-namespace {
-
-struct MappedBlock {
-	hipMemGenericAllocationHandle_t handle{};
-	uint64_t size = 0;
-};
-std::mutex g_mappedMutex;
-std::unordered_map<uintptr_t, MappedBlock> g_mapped;
-
-// Physical allocation of at least numBytes on the current device, mapped
-// read-write at a reserved address; nullptr if any step fails (nothing is
-// left allocated then).
-void* mappedAlloc(uint64_t numBytes)
-{
-	int dev = 0;
-	if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-	hipMemAllocationProp prop{};
-	prop.type = hipMemAllocationTypePinned;
-	prop.location.type = hipMemLocationTypeDevice;
-	prop.location.id = dev;
-	size_t granularity = 0;
-	if (hipMemGetAllocationGranularity(&granularity, &prop, hipMemAllocationGranularityMinimum) != hipSuccess ||
-	    granularity == 0)
-		return nullptr;
-	const uint64_t size = (numBytes + granularity - 1) / granularity * granularity;
-	hipMemGenericAllocationHandle_t handle{};
-	if (hipMemCreate(&handle, size, &prop, 0) != hipSuccess) return nullptr;
-	void* va = nullptr;
-	if (hipMemAddressReserve(&va, size, 0, nullptr, 0) == hipSuccess) {
-		if (hipMemMap(va, size, 0, handle, 0) == hipSuccess) {
-			hipMemAccessDesc access{};
-			access.location = prop.location;
-			access.flags = hipMemAccessFlagsProtReadWrite;
-			if (hipMemSetAccess(va, size, &access, 1) == hipSuccess) {
-				std::lock_guard<std::mutex> lock(g_mappedMutex);
-				g_mapped[reinterpret_cast<uintptr_t>(va)] = MappedBlock{handle, size};
-				return va;
-			}
-			(void)hipMemUnmap(va, size);
-		}
-		(void)hipMemAddressFree(va, size);
-	}
-	(void)hipMemRelease(handle);
-	return nullptr;
-}
-
-}  // namespace
-
-hipError_t deviceAlloc(void** ptr, uint64_t numBytes)
-{
-	*ptr = nullptr;
-	if (numBytes >= kMappedAllocMinBytes) {
-		*ptr = mappedAlloc(numBytes);
-		(void)hipGetLastError();  // a failed step above falls back to hipMalloc
-		if (*ptr) return hipSuccess;
-	}
-	return hipMalloc(ptr, numBytes);
-}
-
-hipError_t deviceFree(void* ptr)
-{
-	if (!ptr) return hipSuccess;
-	MappedBlock b;
-	bool mapped = false;
-	{
-		std::lock_guard<std::mutex> lock(g_mappedMutex);
-		auto it = g_mapped.find(reinterpret_cast<uintptr_t>(ptr));
-		if (it != g_mapped.end()) {
-			b = it->second;
-			g_mapped.erase(it);
-			mapped = true;
-		}
-	}
-	if (!mapped) return hipFree(ptr);
-	hipError_t e = hipDeviceSynchronize();  // hipFree's guarantee: no kernel still uses the range
-	const hipError_t u = hipMemUnmap(ptr, b.size);
-	const hipError_t a = hipMemAddressFree(ptr, b.size);
-	const hipError_t r = hipMemRelease(b.handle);
-	if (e == hipSuccess) e = u;
-	if (e == hipSuccess) e = a;
-	if (e == hipSuccess) e = r;
-	return e;
 }
 
 // Grid-stride fill of 32-bit words; used instead of hipMemsetAsync so that
@@ -201,7 +116,7 @@ CADR_API int cadrDeviceMalloc(uint64_t numBytes, uint64_t* deviceAddress)
 	*deviceAddress = 0;
 	if (numBytes == 0) return ok();
 	void* p = nullptr;
-	hipError_t e = deviceAlloc(&p, numBytes);
+	hipError_t e = hipMalloc(&p, numBytes);
 	if (e != hipSuccess) {
 		(void)hipGetLastError();
 		return setError(CADR_ERROR_OUT_OF_RESOURCES,
@@ -214,7 +129,7 @@ CADR_API int cadrDeviceMalloc(uint64_t numBytes, uint64_t* deviceAddress)
 CADR_API int cadrDeviceFree(uint64_t deviceAddress)
 {
 	if (deviceAddress == 0) return ok();
-	CADR_CHECK_HIP(deviceFree(reinterpret_cast<void*>(deviceAddress)), "cadrDeviceFree");
+	CADR_CHECK_HIP(hipFree(reinterpret_cast<void*>(deviceAddress)), "hipFree");
 	return ok();
 }
 

@@ -49,12 +49,12 @@ class OutputPool {
 public:
 	~OutputPool()
 	{
-		if (_base) (void)cadr::deviceFree(reinterpret_cast<void*>(_base));
+		if (_base) (void)hipFree(reinterpret_cast<void*>(_base));
 	}
 	int init(uint64_t bytes)
 	{
 		void* p = nullptr;
-		if (cadr::deviceAlloc(&p, bytes) != hipSuccess) {
+		if (hipMalloc(&p, bytes) != hipSuccess) {
 			(void)hipGetLastError();
 			return cadr::setError(CADR_ERROR_OUT_OF_RESOURCES, "serve: cannot allocate the output budget");
 		}
@@ -200,13 +200,13 @@ cadrServer::~cadrServer()
 		if (s.stream) (void)hipStreamDestroy(s.stream);
 		if (s.done) (void)hipEventDestroy(s.done);
 		if (s.ws) cadrCullWorkspaceDestroy(s.ws);
-		(void)cadr::deviceFree(reinterpret_cast<void*>(s.dViews));
-		(void)cadr::deviceFree(reinterpret_cast<void*>(s.dJobs));
+		(void)hipFree(reinterpret_cast<void*>(s.dViews));
+		(void)hipFree(reinterpret_cast<void*>(s.dJobs));
 		if (s.hViews) (void)hipHostFree(s.hViews);
 		if (s.hJobs) (void)hipHostFree(s.hJobs);
 		if (s.hCounts) (void)hipHostFree(s.hCounts);
 	}
-	for (SceneRec& sc : scenes) (void)cadr::deviceFree(reinterpret_cast<void*>(sc.tiles));
+	for (SceneRec& sc : scenes) (void)hipFree(reinterpret_cast<void*>(sc.tiles));
 }
 
 void cadrServer::complete(Slot& s, int status, uint64_t t)
@@ -484,7 +484,7 @@ CADR_API int cadrServerAddScene(cadrServer* s, const cadrServerScene* scene, uin
 	rc = cadrBuildCullTiles(rec.counts.data(), scene->numStateSets, s->tileSize, tiles.data(), n, &n);
 	if (rc != CADR_SUCCESS) return rc;
 	void* d = nullptr;
-	CADR_CHECK_HIP(cadr::deviceAlloc(&d, std::max<size_t>(n, 1) * sizeof(cadrCullTile)), "device allocation (tiles)");
+	CADR_CHECK_HIP(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(cadrCullTile)), "hipMalloc(tiles)");
 	rec.tiles = reinterpret_cast<uint64_t>(d);
 	CADR_CHECK_HIP(hipMemcpy(d, tiles.data(), n * sizeof(cadrCullTile), hipMemcpyHostToDevice), "hipMemcpy(tiles)");
 	rec.numTiles = n;
@@ -508,9 +508,9 @@ CADR_API int cadrServerStart(cadrServer* s)
 		const int rc = cadrCullWorkspaceCreate(uint64_t(V) * s->maxTiles, V, &sl.ws);
 		if (rc != CADR_SUCCESS) return rc;
 		void* p = nullptr;
-		CADR_CHECK_HIP(cadr::deviceAlloc(&p, uint64_t(V) * sizeof(cadrView)), "device allocation (views)");
+		CADR_CHECK_HIP(hipMalloc(&p, uint64_t(V) * sizeof(cadrView)), "hipMalloc(views)");
 		sl.dViews = reinterpret_cast<uint64_t>(p);
-		CADR_CHECK_HIP(cadr::deviceAlloc(&p, uint64_t(V) * sizeof(cadrCullJob)), "device allocation (jobs)");
+		CADR_CHECK_HIP(hipMalloc(&p, uint64_t(V) * sizeof(cadrCullJob)), "hipMalloc(jobs)");
 		sl.dJobs = reinterpret_cast<uint64_t>(p);
 		CADR_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hViews), uint64_t(V) * sizeof(cadrView),
 		                             hipHostMallocDefault), "hipHostMalloc(views)");

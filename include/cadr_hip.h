@@ -65,13 +65,6 @@ int cadrSetDevice(int deviceIndex);
 int cadrGetDeviceCount(int* count);
 /* Number of compute units of the current device (256 on MI355X). */
 int cadrGetComputeUnitCount(int* count);
-/* Device memory.  From 64 MiB on it is a physical allocation mapped into a
- * reserved address range (hipMemCreate + hipMemMap; the cull's scattered
- * command stores ran 2-10% faster there than in hipMalloc memory),
- * below that, or when that path fails, hipMalloc memory.  Either is
- * ordinary device memory for copies and kernels; it has no IPC handle.
- * cadrDeviceFree takes only addresses cadrDeviceMalloc returned and, like
- * hipFree, waits for the device first. */
 int cadrDeviceMalloc(uint64_t numBytes, uint64_t* deviceAddress);
 int cadrDeviceFree(uint64_t deviceAddress);
 int cadrHostMallocPinned(uint64_t numBytes, void** hostPointer);
